@@ -1,0 +1,173 @@
+/*
+ * spx_capi.h -- extern "C" ABI of libspx.so, the MI355X (gfx950) FP64 CSR
+ * SpMV library.  Plain pointers and sizes only.  The drop-in C++ functions
+ * of spmv_kernel.h are one-line wrappers over the spx_dropin_* calls here;
+ * an FFI (ctypes, cgo, JNI) binds these names directly.
+ *
+ * Two layers:
+ *  1. spx_dropin_*: the reference's entry points with identical arguments and
+ *     return codes (host pointers in, host y out; stateless per call).
+ *  2. spx_plan_*: a device-resident handle (convert once, execute many) that
+ *     the drop-in layer is built on.  x and y are DEVICE pointers here.
+ *
+ * Status codes of the plan layer: SPX_OK (0) or a negative SPX_E* value; the
+ * text of the last error on the calling thread is in spx_last_error().
+ */
+#ifndef SPX_CAPI_H
+#define SPX_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPX_OK = 0,
+  SPX_EARG = -1,     /* invalid argument */
+  SPX_EMEM = -2,     /* device allocation failed */
+  SPX_EDEV = -3,     /* HIP runtime / launch failure */
+  SPX_EGUARD = -4,   /* footprint above 0.8 of free device memory */
+  SPX_ERANGE = -5    /* size outside what the kernels support (nnz >= 2^31) */
+};
+
+/* Work decomposition ("mode") of a plan. */
+enum {
+  SPX_MODE_ROWS = 0,   /* contiguous row blocks (spMV_mgpu_baseline) */
+  SPX_MODE_NNZ = 1,    /* nnz-balanced partitions, one HIP stream each (v1) */
+  SPX_MODE_QUEUE = 2   /* nb-nonzero tasks drained from a device work queue
+                          by persistent workgroups on q streams (v2) */
+};
+
+/* Kernel versions, as in the reference's `kernel` argument. */
+enum {
+  SPX_KERNEL_VECTOR = 1, /* wavefront/lane-group per row CSR-vector */
+  SPX_KERNEL_MERGE = 2,  /* merge-path over (rows + nonzeros) */
+  SPX_KERNEL_TILE = 3    /* CSR5-style 64-lane tiles with bit-flag segments */
+};
+
+/* Value storage. */
+enum {
+  SPX_PREC_F64 = 0,      /* FP64 values, FP64 x */
+  SPX_PREC_F32 = 1       /* FP32 values, x as FP32 (hi, lo) pairs, FP64
+                            accumulation, carries and beta update */
+};
+
+typedef struct spx_plan_desc {
+  int m, n;
+  int64_t nnz;
+  const int64_t *rowptr64;  /* host row pointer (m+1), or NULL ... */
+  const int32_t *rowptr32;  /* ... host int32 row pointer (m+1) */
+  const int32_t *col;       /* host column indices (nnz) */
+  const double *val;        /* host values (nnz) */
+  int kernel;               /* SPX_KERNEL_* */
+  int mode;                 /* SPX_MODE_* */
+  int parts;                /* ROWS/NNZ: number of partitions (streams), >= 1 */
+  int64_t nb;               /* QUEUE: task size in nonzeros (> 0) */
+  int queues;               /* QUEUE: streams (copy_of_workspace), >= 1 */
+  int sigma;                /* TILE: elements per lane, 4..32 (0 = auto) */
+  int lanes;                /* VECTOR: lanes per row, power of 2 <= 64 (0 = auto) */
+  int items;                /* MERGE: merge items per thread, 1..16 (0 = auto) */
+  int precision;            /* SPX_PREC_* */
+  int flags;                /* SPX_FLAG_* bits */
+} spx_plan_desc;
+
+/* desc.flags: the row pointer, column and value arrays are DEVICE memory
+ * (the plan copies them; the caller keeps ownership). */
+#define SPX_FLAG_DEVICE_PTRS 1
+
+typedef struct spx_plan *spx_plan_t;
+
+/* Initialise the desc with defaults (kernel 2, one partition, auto tiles). */
+void spx_plan_desc_init(spx_plan_desc *d);
+
+/* Upload A to device memory and run the format conversion on the device.
+ * Returns SPX_OK and *out, or an error (nothing leaks on failure). */
+int spx_plan_create(spx_plan_t *out, const spx_plan_desc *d);
+int spx_plan_destroy(spx_plan_t p);
+
+/* y = alpha*A*x + beta*y on device vectors x (n) and y (m), enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream).  beta == 0 never reads y.
+ * Partitions run on plan-owned streams that fork from and join back to
+ * `stream`.  Asynchronous: returns after enqueueing. */
+int spx_plan_execute(spx_plan_t p, double alpha, const double *x, double beta,
+                     double *y, void *stream);
+
+/* Multi-RHS: Y(:,j) = alpha*A*X(:,j) + beta*Y(:,j) for j < k, X (n-by-k) and
+ * Y (m-by-k) column-major with leading dimensions ldx >= n, ldy >= m. */
+int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
+                           int64_t ldx, double beta, double *Y, int64_t ldy,
+                           void *stream);
+
+/* Plan facts: writes up to `cap` int64 values:
+ * [0] m [1] n [2] nnz [3] kernel [4] mode [5] parts [6] sigma [7] lanes
+ * [8] items [9] units (workgroup work items) [10] carries [11] tiles
+ * [12] device bytes held [13] precision [14] queues. Returns the count. */
+int spx_plan_info(spx_plan_t p, int64_t *out, int cap);
+
+/* Copy plan internals to host for bit-exact checks.  `what`:
+ * 0 partitions: 6 int64 per partition (start_idx, end_idx, start_row,
+ *   end_row, start_flag, end_flag);
+ * 1 tile pointers (uint32, per partition p_i + 1 entries, concatenated);
+ * 2 tile flags (uint32, 64 per tile, concatenated);
+ * 3 tile-layout column indices (int32); 4 tile-layout values (double);
+ * 5 merge-path unit coordinates (int64 pairs: row, nonzero per boundary).
+ * Returns the number of BYTES available; copies min(bytes, cap) into dst
+ * when dst != NULL. */
+int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap);
+
+/* Average device time (ms) of each of `reps` executions after `warmup`
+ * untimed ones, measured with HIP events on the plan's launch stream.
+ * out_ms[0] = whole execute (events around each execute), out_ms[1] = the
+ * dominant SpMV kernel alone (partition kernels, fixup excluded).
+ * If flush_bytes > 0 a buffer of that size is overwritten before every
+ * timed execute so L2/MALL start cold (the flush itself is not timed). */
+int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta,
+                  double *y, int warmup, int reps, int64_t flush_bytes,
+                  double *out_ms);
+
+/* ---- reference drop-in entry points (host pointers) ----------------------
+ * Arguments and return codes as SparseBLAS/include/spmv_kernel.h:4-34. */
+int spx_dropin_baseline(int m, int n, long long nnz, double *alpha,
+                        double *csrVal, long long *csrRowPtr,
+                        int *csrColIndex, double *x, double *beta, double *y,
+                        int ngpu);
+int spx_dropin_v1(int m, int n, long long nnz, double *alpha, double *csrVal,
+                  long long *csrRowPtr, int *csrColIndex, double *x,
+                  double *beta, double *y, int ngpu, int kernel);
+int spx_dropin_v2(int m, int n, long long nnz, double *alpha, double *csrVal,
+                  long long *csrRowPtr, int *csrColIndex, double *x,
+                  double *beta, double *y, int ngpu, int kernel, long long nb,
+                  int copy_of_workspace);
+/* Device pointers, int32 row pointer (SparseBLAS/src/csr5_kernel.cu:12-32). */
+int spx_dropin_csr5_kernel(int m, int n, int nnz, double *alpha,
+                           double *csrVal, int *csrRowPtr, int *csrColIndex,
+                           double *x, double *beta, double *y);
+int spx_get_row_from_index(int n, long long *a, long long idx);
+double spx_get_time(void);
+double spx_get_gpu_availble_mem(int ngpu);
+
+/* Number of stream partitions the v1 drop-in uses at ngpu = 1 (default 1;
+ * the schedule selector sets it).  Returns the previous value. */
+int spx_set_v1_streams(int parts);
+
+/* Host-side seam for tests: when bytes >= 0, the memory guard uses it as the
+ * free device memory instead of hipMemGetInfo (-1 restores the query). */
+void spx_debug_set_free_mem(long long bytes);
+
+/* ---- device memory helpers (so FFI callers need no HIP bindings) -------- */
+void *spx_malloc(int64_t bytes);
+int spx_free(void *p);
+int spx_memcpy_h2d(void *dst, const void *src, int64_t bytes);
+int spx_memcpy_d2h(void *dst, const void *src, int64_t bytes);
+int spx_memcpy_d2d(void *dst, const void *src, int64_t bytes);
+int spx_synchronize(void);
+int spx_device_count(void);
+
+const char *spx_last_error(void);
+const char *spx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

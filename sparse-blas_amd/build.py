@@ -1,0 +1,85 @@
+"""Build libspx.so (gfx950 HIP kernels + C ABI + drop-in C++ API) and
+libspxgen.so (host-only synthetic matrix generators) in-tree.
+
+    python sparse-blas_amd/build.py          # both libraries
+    python sparse-blas_amd/build.py --lib    # libspx.so only
+
+hipcc cross-compiles for gfx950 without a GPU.  Packed FP32 instructions are
+disabled for all device code (-packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SPX_OFFLOAD_ARCH", "gfx950")
+
+LIB = os.path.join(HERE, "libspx.so")
+GENLIB = os.path.join(HERE, "libspxgen.so")
+
+DEVICE_SOURCES = ["kernels.hip", "plan.cpp", "dropin.cpp"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "-Wno-unused-result", "-ffp-contract=fast",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+
+def _run(cmd):
+    t = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if err.strip():
+        sys.stderr.write(err + "\n")
+    return time.time() - t
+
+
+def build_lib():
+    objs = []
+    for src in DEVICE_SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        _run([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+def build_gen():
+    tmp = GENLIB + ".tmp"
+    _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall", "-o", tmp,
+          os.path.join(CSRC, "matgen.cpp")])
+    os.replace(tmp, GENLIB)
+    return GENLIB
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true")
+    ap.add_argument("--gen", action="store_true")
+    a = ap.parse_args(argv)
+    both = not (a.lib or a.gen)
+    if a.gen or both:
+        print("built", build_gen())
+    if a.lib or both:
+        print("built", build_lib())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,243 @@
+// dropin.cpp -- the reference's entry points (SparseBLAS/include/spmv_kernel.h)
+// on one MI355X.  Each call is stateless like the reference's: it builds a
+// device plan from the caller's host arrays, copies x (and y when beta != 0)
+// in, executes, copies y out and frees everything.  Return codes follow the
+// reference: 0 ok, -1 bad argument / memory guard / compute failure, 1 device
+// allocation or copy failure.
+#include <hip/hip_runtime.h>
+#include <sys/time.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+
+#include "../../include/spmv_kernel.h"
+#include "../../include/spx_capi.h"
+
+namespace {
+
+std::atomic<long long> g_free_override{-1};
+std::atomic<int> g_v1_streams{1};
+
+// Footprint of one partition as the reference's guard counts it
+// (SB/src/dspmv_mgpu_v1.cu:107-112): values, columns, int32 row pointer, x, y.
+double footprint_gb(long long nnz, long long rows, int n) {
+  long long b = nnz * (long long)sizeof(double) + nnz * (long long)sizeof(int) +
+                (rows + 1) * (long long)sizeof(int) + (long long)n * sizeof(double) +
+                rows * (long long)sizeof(double);
+  return (double)b / 1e9;
+}
+
+int code_of(int rc) {
+  if (rc == SPX_OK) return 0;
+  if (rc == SPX_EMEM) return 1;
+  return -1;
+}
+
+// Host arrays in, host y out; one plan.
+int run_host(const spx_plan_desc &d, double alpha, const double *x, double beta, double *y) {
+  spx_plan_t plan = nullptr;
+  int rc = spx_plan_create(&plan, &d);
+  if (rc != SPX_OK) {
+    std::fprintf(stderr, "spx: plan creation failed: %s\n", spx_last_error());
+    return code_of(rc);
+  }
+  double *dx = nullptr, *dy = nullptr;
+  int ret = 0;
+  if (d.m > 0) {
+    dx = (double *)spx_malloc((int64_t)std::max(d.n, 1) * 8);
+    dy = (double *)spx_malloc((int64_t)d.m * 8);
+    if (!dx || !dy) {
+      ret = 1;
+    } else if ((d.n > 0 && spx_memcpy_h2d(dx, x, (int64_t)d.n * 8) != SPX_OK) ||
+               (beta != 0.0 && spx_memcpy_h2d(dy, y, (int64_t)d.m * 8) != SPX_OK)) {
+      ret = 1;
+    } else if (spx_plan_execute(plan, alpha, dx, beta, dy, nullptr) != SPX_OK || spx_synchronize() != SPX_OK) {
+      std::fprintf(stderr, "spx: execute failed: %s\n", spx_last_error());
+      ret = -1;
+    } else if (spx_memcpy_d2h(y, dy, (int64_t)d.m * 8) != SPX_OK) {
+      ret = 1;
+    }
+  }
+  if (dx) spx_free(dx);
+  if (dy) spx_free(dy);
+  spx_plan_destroy(plan);
+  return ret;
+}
+
+bool common_args_ok(int m, int n, long long nnz, double *alpha, double *beta, double *y, int ngpu) {
+  if (ngpu != 1) {
+    std::fprintf(stderr, "spx: ngpu must be 1 (this library drives one MI355X)\n");
+    return false;
+  }
+  if (m < 0 || n < 0 || nnz < 0 || !alpha || !beta || (m > 0 && !y)) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void spx_debug_set_free_mem(long long bytes) { g_free_override = bytes; }
+
+int spx_set_v1_streams(int parts) {
+  int old = g_v1_streams.load();
+  if (parts >= 1) g_v1_streams = parts;
+  return old;
+}
+
+int spx_get_row_from_index(int n, long long *a, long long idx) {
+  // SB/src/spmv_helper.cu:16-39, behaviour kept exactly (including the
+  // early return on the first probe equal to idx)
+  int l = 0, r = n;
+  while (l < r - 1) {
+    int mid = l + (r - l) / 2;
+    if (idx < a[mid])
+      r = mid;
+    else if (idx > a[mid])
+      l = mid;
+    else
+      return mid;
+  }
+  if (idx == a[l]) return l;
+  if (idx == a[r]) return r;
+  return l;
+}
+
+double spx_get_time(void) {
+  // SB/src/spmv_helper.cu:41-48
+  struct timeval tp;
+  gettimeofday(&tp, nullptr);
+  return (double)tp.tv_sec + (double)tp.tv_usec / 1e6;
+}
+
+double spx_get_gpu_availble_mem(int ngpu) {
+  // SB/src/spmv_helper.cu:51-76: minimum free memory (GB) over the GPUs
+  long long ov = g_free_override.load();
+  if (ov >= 0) return (double)ov / 1e9;
+  double mn = std::numeric_limits<double>::max();
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (int d = 0; d < ngpu; d++) {
+    size_t fr = 0, tot = 0;
+    if (hipSetDevice(d) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      mn = 0;
+      break;
+    }
+    mn = std::min(mn, (double)fr / 1e9);
+  }
+  (void)hipSetDevice(cur);
+  return mn;
+}
+
+int spx_dropin_baseline(int m, int n, long long nnz, double *alpha, double *csrVal, long long *csrRowPtr,
+                        int *csrColIndex, double *x, double *beta, double *y, int ngpu) {
+  if (!common_args_ok(m, n, nnz, alpha, beta, y, ngpu)) return -1;
+  // SB/src/dspmv_mgpu_baseline.cu:70-79
+  if (footprint_gb(nnz, m, n) > 0.8 * spx_get_gpu_availble_mem(ngpu)) return -1;
+  spx_plan_desc d;
+  spx_plan_desc_init(&d);
+  d.m = m; d.n = n; d.nnz = nnz;
+  d.rowptr64 = reinterpret_cast<const int64_t *>(csrRowPtr);
+  d.col = csrColIndex; d.val = csrVal;
+  d.kernel = SPX_KERNEL_VECTOR;
+  d.mode = SPX_MODE_ROWS;
+  d.parts = 1;
+  return run_host(d, *alpha, x, *beta, y);
+}
+
+int spx_dropin_v1(int m, int n, long long nnz, double *alpha, double *csrVal, long long *csrRowPtr,
+                  int *csrColIndex, double *x, double *beta, double *y, int ngpu, int kernel) {
+  if (!common_args_ok(m, n, nnz, alpha, beta, y, ngpu)) return -1;
+  if (kernel < 1 || kernel > 3) return -1;
+  // SB/src/dspmv_mgpu_v1.cu:106-117 (one partition per GPU at ngpu = 1)
+  if (footprint_gb(nnz, m, n) > 0.8 * spx_get_gpu_availble_mem(ngpu)) return -1;
+  spx_plan_desc d;
+  spx_plan_desc_init(&d);
+  d.m = m; d.n = n; d.nnz = nnz;
+  d.rowptr64 = reinterpret_cast<const int64_t *>(csrRowPtr);
+  d.col = csrColIndex; d.val = csrVal;
+  d.kernel = kernel;
+  d.mode = SPX_MODE_NNZ;
+  d.parts = g_v1_streams.load();
+  return run_host(d, *alpha, x, *beta, y);
+}
+
+int spx_dropin_v2(int m, int n, long long nnz, double *alpha, double *csrVal, long long *csrRowPtr,
+                  int *csrColIndex, double *x, double *beta, double *y, int ngpu, int kernel, long long nb,
+                  int copy_of_workspace) {
+  if (!common_args_ok(m, n, nnz, alpha, beta, y, ngpu)) return -1;
+  if (kernel < 1 || kernel > 3) return -1;
+  const int q = copy_of_workspace;
+  // SB/src/dspmv_mgpu_v2.cu:43-46
+  if (q <= 0 || nb <= 0) return -1;
+  long long cap = (long long)(0.8 * spx_get_gpu_availble_mem(ngpu) * 1e9 /
+                              (double)(sizeof(double) + sizeof(int) + sizeof(int))) / q;
+  nb = std::min(nb, cap);
+  if (nb <= 0) return -1;
+  spx_plan_desc d;
+  spx_plan_desc_init(&d);
+  d.m = m; d.n = n; d.nnz = nnz;
+  d.rowptr64 = reinterpret_cast<const int64_t *>(csrRowPtr);
+  d.col = csrColIndex; d.val = csrVal;
+  d.kernel = kernel;
+  d.mode = SPX_MODE_QUEUE;
+  d.nb = nb;
+  d.queues = q;
+  return run_host(d, *alpha, x, *beta, y);
+}
+
+int spx_dropin_csr5_kernel(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr, int *csrColIndex,
+                           double *x, double *beta, double *y) {
+  // SB/src/csr5_kernel.cu:12-32 on device pointers; alpha and beta honoured
+  if (m < 0 || n < 0 || nnz < 0 || !alpha || !beta) return -1;
+  spx_plan_desc d;
+  spx_plan_desc_init(&d);
+  d.m = m; d.n = n; d.nnz = nnz;
+  d.rowptr32 = csrRowPtr;
+  d.col = csrColIndex; d.val = csrVal;
+  d.kernel = SPX_KERNEL_TILE;
+  d.mode = SPX_MODE_NNZ;
+  d.parts = 1;
+  d.flags = SPX_FLAG_DEVICE_PTRS;
+  spx_plan_t plan = nullptr;
+  int rc = spx_plan_create(&plan, &d);
+  if (rc != SPX_OK) return code_of(rc);
+  rc = spx_plan_execute(plan, *alpha, x, *beta, y, nullptr);
+  if (rc == SPX_OK) rc = spx_synchronize();
+  spx_plan_destroy(plan);
+  return rc == SPX_OK ? 0 : -1;
+}
+
+}  // extern "C"
+
+// ---- C++ linkage, as declared by the reference header ---------------------------
+
+int csr5_kernel(int m, int n, int nnz, double *alpha, double *csrVal, int *csrRowPtr, int *csrColIndex, double *x,
+                double *beta, double *y) {
+  return spx_dropin_csr5_kernel(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y);
+}
+
+int spMV_mgpu_baseline(int m, int n, long long nnz, double *alpha, double *csrVal, long long *csrRowPtr,
+                       int *csrColIndex, double *x, double *beta, double *y, int ngpu) {
+  return spx_dropin_baseline(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu);
+}
+
+int spMV_mgpu_v1(int m, int n, long long nnz, double *alpha, double *csrVal, long long *csrRowPtr, int *csrColIndex,
+                 double *x, double *beta, double *y, int ngpu, int kernel) {
+  return spx_dropin_v1(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu, kernel);
+}
+
+int spMV_mgpu_v2(int m, int n, long long nnz, double *alpha, double *csrVal, long long *csrRowPtr, int *csrColIndex,
+                 double *x, double *beta, double *y, int ngpu, int kernel, long long nb, int copy_of_workspace) {
+  return spx_dropin_v2(m, n, nnz, alpha, csrVal, csrRowPtr, csrColIndex, x, beta, y, ngpu, kernel, nb,
+                       copy_of_workspace);
+}
+
+int get_row_from_index(int n, long long *a, long long idx) { return spx_get_row_from_index(n, a, idx); }
+
+double get_time() { return spx_get_time(); }
+
+double get_gpu_availble_mem(int ngpu) { return spx_get_gpu_availble_mem(ngpu); }

@@ -1,0 +1,640 @@
+// plan.cpp -- device-resident SpMV plans behind the extern "C" ABI.
+//
+// spx_plan_create uploads A once, partitions it on the device (nnz-balanced
+// partitions -> HIP streams, each partition -> workgroup units) and runs the
+// kernel-specific conversion (merge-path unit coordinates, or CSR5-style
+// tiles).  spx_plan_execute then only launches the SpMV kernels and the
+// deterministic carry fix-up.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/spx_capi.h"
+#include "spx_internal.h"
+
+using namespace spx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV,                     \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+int default_lanes(double mean) {
+  // lanes per row for the CSR-vector kernel from the mean row length
+  if (mean <= 2) return 2;
+  if (mean <= 4) return 4;
+  if (mean <= 8) return 8;
+  if (mean <= 16) return 16;
+  if (mean <= 32) return 32;
+  return 64;
+}
+
+int tile_sigma_auto(int m, int64_t nnz) {
+  // SB/include/anonymouslib_cuda.h:297-313
+  int64_t q = m > 0 ? nnz / m : 0;
+  if (q <= 4) return 4;
+  if (q <= 32) return (int)q;
+  if (q <= 256) return 32;
+  return 6;
+}
+
+}  // namespace
+
+struct spx_plan {
+  int m = 0, n = 0;
+  int64_t nnz = 0;
+  int kernel = 2, mode = SPX_MODE_NNZ, nparts = 1, queues = 1;
+  int sigma = 0, lanes = 0, items = 0, precision = SPX_PREC_F64;
+  int64_t nb = 0;
+
+  int32_t *d_rowptr = nullptr;
+  int32_t *d_col = nullptr;      // CSR columns, or tile layout for kernel 3
+  double *d_val = nullptr;       // FP64 values (CSR or tile layout)
+  float *d_val32 = nullptr;      // FP32 values (precision form)
+  XPair *d_xhl = nullptr;        // precision form: split x
+  Part *d_parts = nullptr;
+  std::vector<Part> parts;
+  Unit *d_units = nullptr;
+  int64_t nunits = 0;
+  Coord *d_coords = nullptr;
+  int64_t ncoords = 0;
+  uint32_t *d_tp = nullptr, *d_tf = nullptr;
+  int64_t ntiles = 0, ntp = 0;
+  int32_t *d_crow = nullptr;
+  double *d_cval = nullptr;
+  int64_t ncarry = 0;
+  bool any_carry = false;
+  int32_t *d_empty = nullptr;
+  int64_t nempty = 0;
+  int *d_queue = nullptr;
+
+  struct Launch {
+    int stream;
+    int64_t ubeg, ucnt;
+    int grid;
+  };
+  std::vector<Launch> launches;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork = nullptr;
+  int64_t bytes = 0;
+
+  ~spx_plan() {
+    for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
+                    (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
+                    (void *)d_crow, (void *)d_cval, (void *)d_empty, (void *)d_queue})
+      if (p) (void)hipFree(p);
+    for (auto s : streams) (void)hipStreamDestroy(s);
+    for (auto e : done) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+  }
+
+  template <typename T>
+  int alloc(T **p, int64_t count) {
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV,
+                  std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    bytes += (int64_t)sizeof(T) * count;
+    return SPX_OK;
+  }
+};
+
+extern "C" {
+
+void spx_plan_desc_init(spx_plan_desc *d) {
+  std::memset(d, 0, sizeof(*d));
+  d->kernel = SPX_KERNEL_MERGE;
+  d->mode = SPX_MODE_NNZ;
+  d->parts = 1;
+  d->queues = 1;
+}
+
+const char *spx_last_error(void) { return g_last_error.c_str(); }
+const char *spx_version(void) { return "spx 0.1 gfx950"; }
+
+static int plan_build(spx_plan *P, const spx_plan_desc *d) {
+  const int m = d->m, n = d->n;
+  const int64_t nnz = d->nnz;
+  if (m < 0 || n < 0 || nnz < 0) return fail(SPX_EARG, "negative size");
+  if (nnz >= ((int64_t)1 << 31) - 1) return fail(SPX_ERANGE, "nnz must be below 2^31 per plan");
+  if (d->kernel < 1 || d->kernel > 3) return fail(SPX_EARG, "kernel must be 1, 2 or 3");
+  if (d->mode < 0 || d->mode > 2) return fail(SPX_EARG, "bad mode");
+  if (d->precision != SPX_PREC_F64 && d->precision != SPX_PREC_F32) return fail(SPX_EARG, "bad precision");
+  if (m > 0 && !d->rowptr64 && !d->rowptr32) return fail(SPX_EARG, "missing row pointer");
+  if (nnz > 0 && (!d->col || !d->val)) return fail(SPX_EARG, "missing col/val");
+  P->m = m;
+  P->n = n;
+  P->nnz = nnz;
+  P->kernel = d->kernel;
+  P->mode = d->mode;
+  P->precision = d->precision;
+  P->queues = d->queues > 0 ? d->queues : 1;
+  P->nb = d->nb;
+  if (m == 0) return SPX_OK;
+
+  // host int32 row pointer (validated: monotone, ends at nnz)
+  const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
+  std::vector<int64_t> rp64;
+  std::vector<int32_t> rp32;
+  const int64_t *src64 = d->rowptr64;
+  const int32_t *src32 = d->rowptr32;
+  if (on_dev) {
+    if (src64) {
+      rp64.resize(m + 1);
+      HIP_TRY(hipMemcpy(rp64.data(), src64, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost));
+      src64 = rp64.data();
+    } else {
+      rp32.resize(m + 1);
+      HIP_TRY(hipMemcpy(rp32.data(), src32, sizeof(int32_t) * (m + 1), hipMemcpyDeviceToHost));
+      src32 = rp32.data();
+    }
+  }
+  std::vector<int32_t> rp(m + 1);
+  for (int i = 0; i <= m; i++) {
+    int64_t v = src64 ? src64[i] : (int64_t)src32[i];
+    if (v < 0 || v > nnz || (i > 0 && v < rp[i - 1])) return fail(SPX_EARG, "row pointer not monotone within [0, nnz]");
+    rp[i] = (int32_t)v;
+  }
+  if (rp[0] != 0 || rp[m] != nnz) return fail(SPX_EARG, "row pointer must start at 0 and end at nnz");
+
+  // partitions
+  int kind = PART_NNZ, np = 1;
+  if (d->mode == SPX_MODE_ROWS) {
+    kind = PART_ROWS;
+    np = std::max(1, std::min(d->parts, m));
+  } else if (d->mode == SPX_MODE_NNZ) {
+    kind = PART_NNZ;
+    np = std::max(1, d->parts);
+    if (nnz > 0 && np > nnz) np = (int)nnz;
+    if (nnz == 0) np = 1;
+  } else {
+    kind = PART_TASKS;
+    if (d->nb <= 0) return fail(SPX_EARG, "nb must be positive");
+    int64_t T = nnz > 0 ? (nnz + d->nb - 1) / d->nb : 1;  // SB/src/dspmv_mgpu_v2.cu:218
+    if (T > (1 << 24)) return fail(SPX_ERANGE, "too many tasks (nb too small)");
+    np = (int)T;
+  }
+  P->nparts = np;
+
+  int rc;
+  if ((rc = P->alloc(&P->d_rowptr, m + 1))) return rc;
+  HIP_TRY(hipMemcpy(P->d_rowptr, rp.data(), sizeof(int32_t) * (m + 1), hipMemcpyHostToDevice));
+  if ((rc = P->alloc(&P->d_parts, np))) return rc;
+  if (nnz == 0) {
+    // every row empty: one partition owning all rows
+    Part q = {};
+    q.s = 0;
+    q.e = -1;
+    q.row_lo = 0;
+    q.row_hi = m;
+    P->parts.assign(1, q);
+  } else {
+    HIP_TRY(launch_partition(P->d_rowptr, m, nnz, np, kind, P->d_parts, 0));
+    P->parts.resize(np);
+    HIP_TRY(hipMemcpy(P->parts.data(), P->d_parts, sizeof(Part) * np, hipMemcpyDeviceToHost));
+  }
+
+  // per-kernel work decomposition
+  const double mean = m > 0 ? (double)nnz / m : 0;
+  if (P->kernel == SPX_KERNEL_VECTOR) {
+    P->lanes = d->lanes > 0 ? d->lanes : default_lanes(mean);
+    if (P->lanes > 64 || (P->lanes & (P->lanes - 1))) return fail(SPX_EARG, "lanes must be a power of two <= 64");
+  } else if (P->kernel == SPX_KERNEL_MERGE) {
+    P->items = d->items > 0 ? d->items : 8;
+    const int ok[] = {2, 4, 6, 8, 12, 16};
+    if (std::find(std::begin(ok), std::end(ok), P->items) == std::end(ok))
+      return fail(SPX_EARG, "items must be one of 2,4,6,8,12,16");
+  } else {
+    P->sigma = d->sigma > 0 ? d->sigma : tile_sigma_auto(m, nnz);
+    if (P->sigma < 1 || P->sigma > 32) return fail(SPX_EARG, "sigma must be in 1..32");
+  }
+  const int tw = kBlock * P->items;
+  int64_t uoff = 0, coff = 0, cooff = 0, toff = 0, tpoff = 0;
+  for (auto &q : P->parts) {
+    const int64_t rows = (int64_t)q.row_hi - q.row_lo;
+    const int64_t nzq = q.e - q.s + 1;
+    q.unit_off = uoff;
+    q.carry_off = coff;
+    if (P->kernel == SPX_KERNEL_VECTOR) {
+      const int G = kBlock / P->lanes;
+      q.units = (rows + G - 1) / G;
+      q.cslot1 = coff + 1;
+      coff += 2;
+    } else if (P->kernel == SPX_KERNEL_MERGE) {
+      const int64_t L = (rows - q.eflag) + nzq;
+      q.units = (L + tw - 1) / tw;
+      q.coord_off = cooff;
+      cooff += q.units + 1;
+      q.cslot1 = coff + 1 + 2 * q.units;
+      coff += 2 + 2 * q.units;
+    } else {
+      const int64_t w = (int64_t)kOmega * P->sigma;
+      q.ntiles = (nzq + w - 1) / w;
+      q.tile_off = toff;
+      q.tp_off = tpoff;
+      toff += q.ntiles;
+      tpoff += q.ntiles + 1;
+      q.units = (q.ntiles + 3) / 4;
+      q.cslot1 = coff + 1 + 2 * q.ntiles;
+      coff += 2 + 2 * q.ntiles;
+    }
+    uoff += q.units;
+  }
+  P->nunits = uoff;
+  P->ncarry = coff;
+  P->ncoords = cooff;
+  P->ntiles = toff;
+  P->ntp = tpoff;
+  HIP_TRY(hipMemcpy(P->d_parts, P->parts.data(), sizeof(Part) * P->parts.size(), hipMemcpyHostToDevice));
+  P->any_carry = false;
+  for (auto &q : P->parts)
+    if (q.sflag || q.eflag) P->any_carry = true;
+  if (P->kernel != SPX_KERNEL_VECTOR && nnz > 0) P->any_carry = true;
+
+  // unit table
+  std::vector<Unit> units(std::max<int64_t>(1, P->nunits));
+  for (size_t pi = 0; pi < P->parts.size(); pi++)
+    for (int64_t u = 0; u < P->parts[pi].units; u++) units[P->parts[pi].unit_off + u] = Unit{(int32_t)pi, (int32_t)u};
+  if ((rc = P->alloc(&P->d_units, P->nunits))) return rc;
+  if (P->nunits)
+    HIP_TRY(hipMemcpy(P->d_units, units.data(), sizeof(Unit) * P->nunits, hipMemcpyHostToDevice));
+
+  if ((rc = P->alloc(&P->d_crow, P->ncarry))) return rc;
+  if ((rc = P->alloc(&P->d_cval, P->ncarry))) return rc;
+  HIP_TRY(launch_fill_i32(P->d_crow, -1, P->ncarry, 0));
+
+  // matrix payload
+  if (P->kernel == SPX_KERNEL_TILE) {
+    const int64_t tot = P->ntiles * kOmega * P->sigma;
+    int32_t *col_tmp = nullptr;
+    double *val_tmp = nullptr;
+    if (nnz > 0) {
+      if (hipMalloc((void **)&col_tmp, sizeof(int32_t) * nnz) != hipSuccess ||
+          hipMalloc((void **)&val_tmp, sizeof(double) * nnz) != hipSuccess) {
+        if (col_tmp) (void)hipFree(col_tmp);
+        return fail(SPX_EMEM, "hipMalloc: staging for tile conversion");
+      }
+      HIP_TRY(hipMemcpy(col_tmp, d->col, sizeof(int32_t) * nnz, hipMemcpyDefault));
+      HIP_TRY(hipMemcpy(val_tmp, d->val, sizeof(double) * nnz, hipMemcpyDefault));
+    }
+    if ((rc = P->alloc(&P->d_tp, P->ntp)) || (rc = P->alloc(&P->d_tf, P->ntiles * kOmega)) ||
+        (rc = P->alloc(&P->d_col, tot))) {
+      (void)hipFree(col_tmp);
+      (void)hipFree(val_tmp);
+      return rc;
+    }
+    if (P->precision == SPX_PREC_F32)
+      rc = P->alloc(&P->d_val32, tot);
+    else
+      rc = P->alloc(&P->d_val, tot);
+    if (rc) {
+      (void)hipFree(col_tmp);
+      (void)hipFree(val_tmp);
+      return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (nnz > 0)
+      e = launch_tile_convert(P->d_rowptr, col_tmp, val_tmp, P->parts.data(), P->d_parts, (int)P->parts.size(),
+                              P->sigma, P->d_tp, P->d_tf, P->d_col, P->d_val, P->d_val32, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(col_tmp);
+    (void)hipFree(val_tmp);
+    HIP_TRY(e);
+    // empty rows are written by their own small kernel
+    std::vector<int32_t> empty;
+    for (int i = 0; i < m; i++)
+      if (rp[i] == rp[i + 1]) empty.push_back(i);
+    P->nempty = (int64_t)empty.size();
+    if ((rc = P->alloc(&P->d_empty, P->nempty))) return rc;
+    if (P->nempty)
+      HIP_TRY(hipMemcpy(P->d_empty, empty.data(), sizeof(int32_t) * P->nempty, hipMemcpyHostToDevice));
+  } else {
+    if ((rc = P->alloc(&P->d_col, nnz))) return rc;
+    if (nnz) HIP_TRY(hipMemcpy(P->d_col, d->col, sizeof(int32_t) * nnz, hipMemcpyDefault));
+    if (P->precision == SPX_PREC_F32) {
+      double *tmp = nullptr;
+      if ((rc = P->alloc(&P->d_val32, nnz))) return rc;
+      if (nnz) {
+        HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * nnz));
+        hipError_t e = hipMemcpy(tmp, d->val, sizeof(double) * nnz, hipMemcpyDefault);
+        if (e == hipSuccess) e = launch_to_f32(tmp, P->d_val32, nnz, 0);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(tmp);
+        HIP_TRY(e);
+      }
+    } else {
+      if ((rc = P->alloc(&P->d_val, nnz))) return rc;
+      if (nnz) HIP_TRY(hipMemcpy(P->d_val, d->val, sizeof(double) * nnz, hipMemcpyDefault));
+    }
+    if (P->kernel == SPX_KERNEL_MERGE) {
+      if ((rc = P->alloc(&P->d_coords, P->ncoords))) return rc;
+      if (nnz > 0 || m > 0)
+        HIP_TRY(launch_merge_coords(P->d_rowptr, P->d_parts, (int)P->parts.size(), P->ncoords, tw, P->d_coords, 0));
+    }
+  }
+  if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
+
+  // launch layout: partitions -> streams
+  int nstreams = 1;
+  if (P->mode == SPX_MODE_QUEUE) {
+    nstreams = std::max(1, std::min(P->queues, (int)P->parts.size()));
+    if ((rc = P->alloc(&P->d_queue, nstreams))) return rc;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int np2 = (int)P->parts.size();
+    for (int c = 0; c < nstreams; c++) {
+      // contiguous task quota per stream (SB/src/dspmv_mgpu_v2.cu:124-126)
+      int t0 = (int)((int64_t)np2 * c / nstreams), t1 = (int)((int64_t)np2 * (c + 1) / nstreams);
+      if (t1 <= t0) continue;
+      int64_t ub = P->parts[t0].unit_off;
+      int64_t ue = P->parts[t1 - 1].unit_off + P->parts[t1 - 1].units;
+      int grid = (int)std::min<int64_t>(std::max<int64_t>(1, ue - ub), (int64_t)cus * 8 / nstreams + 1);
+      P->launches.push_back({c, ub, ue - ub, grid});
+    }
+  } else {
+    // one launch per partition, partitions round-robin over at most 8 streams
+    nstreams = std::min<int>((int)P->parts.size(), 8);
+    for (size_t pi = 0; pi < P->parts.size(); pi++) {
+      const Part &q = P->parts[pi];
+      if (q.units == 0) continue;
+      P->launches.push_back({(int)(pi % nstreams), q.unit_off, q.units, (int)std::min<int64_t>(q.units, 1 << 30)});
+    }
+  }
+  if (nstreams > 1) {
+    P->streams.resize(nstreams);
+    P->done.resize(nstreams);
+    for (int c = 0; c < nstreams; c++) {
+      HIP_TRY(hipStreamCreateWithFlags(&P->streams[c], hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&P->done[c], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return SPX_OK;
+}
+
+int spx_plan_create(spx_plan_t *out, const spx_plan_desc *d) {
+  if (!out || !d) return fail(SPX_EARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<spx_plan> P(new spx_plan());
+  int rc = plan_build(P.get(), d);
+  if (rc != SPX_OK) return rc;
+  *out = P.release();
+  return SPX_OK;
+}
+
+int spx_plan_destroy(spx_plan_t p) {
+  delete p;
+  return SPX_OK;
+}
+
+}  // extern "C"
+
+template <typename V, typename X>
+static hipError_t launch_main(spx_plan *P, const KArgs<V, X> &base, const spx_plan::Launch &L, hipStream_t st) {
+  KArgs<V, X> a = base;
+  a.unit_begin = L.ubeg;
+  a.unit_count = L.ucnt;
+  a.queue = nullptr;
+  int grid = (int)std::min<int64_t>(L.ucnt, 1 << 30);
+  if (P->mode == SPX_MODE_QUEUE) {
+    a.queue = P->d_queue + L.stream;
+    hipError_t e = launch_reset(a.queue, 1, st);
+    if (e != hipSuccess) return e;
+    grid = L.grid;
+  }
+  if constexpr (std::is_same<V, double>::value) {
+    if (P->kernel == SPX_KERNEL_VECTOR) return launch_vector_f64(a, P->lanes, grid, st);
+    if (P->kernel == SPX_KERNEL_MERGE) return launch_merge_f64(a, P->items, grid, st);
+    return launch_tile_f64(a, grid, st);
+  } else {
+    if (P->kernel == SPX_KERNEL_VECTOR) return launch_vector_f32(a, P->lanes, grid, st);
+    if (P->kernel == SPX_KERNEL_MERGE) return launch_merge_f32(a, P->items, grid, st);
+    return launch_tile_f32(a, grid, st);
+  }
+}
+
+template <typename V, typename X>
+static hipError_t run_all(spx_plan *P, const KArgs<V, X> &a, hipStream_t st, hipEvent_t t0, hipEvent_t t1) {
+  hipError_t e = hipSuccess;
+  if (t0) e = hipEventRecord(t0, st);
+  if (e != hipSuccess) return e;
+  if (P->streams.empty()) {
+    for (auto &L : P->launches)
+      if ((e = launch_main(P, a, L, st)) != hipSuccess) return e;
+  } else {
+    if ((e = hipEventRecord(P->fork, st)) != hipSuccess) return e;
+    std::vector<char> used(P->streams.size(), 0);
+    for (auto &L : P->launches) {
+      hipStream_t s = P->streams[L.stream];
+      if (!used[L.stream]) {
+        if ((e = hipStreamWaitEvent(s, P->fork, 0)) != hipSuccess) return e;
+        used[L.stream] = 1;
+      }
+      if ((e = launch_main(P, a, L, s)) != hipSuccess) return e;
+    }
+    for (size_t c = 0; c < P->streams.size(); c++) {
+      if (!used[c]) continue;
+      if ((e = hipEventRecord(P->done[c], P->streams[c])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(st, P->done[c], 0)) != hipSuccess) return e;
+    }
+  }
+  if (t1) e = hipEventRecord(t1, st);
+  return e;
+}
+
+static int execute_impl(spx_plan *P, double alpha, const double *x, double beta, double *y, hipStream_t st,
+                        hipEvent_t t0, hipEvent_t t1) {
+  if (!P) return fail(SPX_EARG, "null plan");
+  if (P->m == 0) return SPX_OK;
+  if (!y || (P->n > 0 && !x)) return fail(SPX_EARG, "null x or y");
+  if (P->kernel == SPX_KERNEL_TILE) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
+  if (P->nnz == 0 && P->kernel == SPX_KERNEL_TILE) return SPX_OK;
+  hipError_t e;
+  if (P->precision == SPX_PREC_F32) {
+    HIP_TRY(launch_split_x(x, P->d_xhl, P->n, st));
+    KArgs<float, XPair> a = {};
+    a.parts = P->d_parts; a.units = P->d_units; a.rowptr = P->d_rowptr; a.col = P->d_col;
+    a.val = P->d_val32; a.x = P->d_xhl; a.alpha = alpha; a.beta = beta; a.y = y;
+    a.crow = P->d_crow; a.cval = P->d_cval; a.coords = P->d_coords; a.tile_ptr = P->d_tp;
+    a.tile_flags = P->d_tf; a.sigma = P->sigma;
+    e = run_all(P, a, st, t0, t1);
+  } else {
+    KArgs<double, double> a = {};
+    a.parts = P->d_parts; a.units = P->d_units; a.rowptr = P->d_rowptr; a.col = P->d_col;
+    a.val = P->d_val; a.x = x; a.alpha = alpha; a.beta = beta; a.y = y;
+    a.crow = P->d_crow; a.cval = P->d_cval; a.coords = P->d_coords; a.tile_ptr = P->d_tp;
+    a.tile_flags = P->d_tf; a.sigma = P->sigma;
+    e = run_all(P, a, st, t0, t1);
+  }
+  HIP_TRY(e);
+  if (P->any_carry) HIP_TRY(launch_fixup(P->d_crow, P->d_cval, P->ncarry, alpha, beta, y, st));
+  return SPX_OK;
+}
+
+extern "C" {
+
+int spx_plan_execute(spx_plan_t p, double alpha, const double *x, double beta, double *y, void *stream) {
+  return execute_impl(p, alpha, x, beta, y, (hipStream_t)stream, nullptr, nullptr);
+}
+
+int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, int64_t ldx, double beta, double *Y,
+                           int64_t ldy, void *stream) {
+  if (!p || k < 0 || ldx < p->n || ldy < p->m) return fail(SPX_EARG, "bad multi-RHS arguments");
+  for (int j = 0; j < k; j++) {
+    int rc = execute_impl(p, alpha, X + (size_t)j * ldx, beta, Y + (size_t)j * ldy, (hipStream_t)stream, nullptr,
+                          nullptr);
+    if (rc) return rc;
+  }
+  return SPX_OK;
+}
+
+int spx_plan_info(spx_plan_t p, int64_t *out, int cap) {
+  if (!p || !out) return 0;
+  int64_t v[] = {p->m, p->n, p->nnz, p->kernel, p->mode, (int64_t)p->parts.size(), p->sigma, p->lanes,
+                 p->items, p->nunits, p->ncarry, p->ntiles, p->bytes, p->precision, p->queues};
+  int c = std::min<int>(cap, (int)(sizeof(v) / sizeof(v[0])));
+  for (int i = 0; i < c; i++) out[i] = v[i];
+  return c;
+}
+
+int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap) {
+  if (!p) return -1;
+  const void *src = nullptr;
+  int64_t bytes = 0;
+  std::vector<int64_t> tmp;
+  switch (what) {
+    case 0:
+      for (auto &q : p->parts) {
+        tmp.push_back(q.s); tmp.push_back(q.e); tmp.push_back(q.r0);
+        tmp.push_back(q.r1); tmp.push_back(q.sflag); tmp.push_back(q.eflag);
+      }
+      bytes = (int64_t)tmp.size() * 8;
+      if (dst) std::memcpy(dst, tmp.data(), (size_t)std::min(bytes, cap));
+      return bytes;
+    case 1: src = p->d_tp; bytes = p->ntp * 4; break;
+    case 2: src = p->d_tf; bytes = p->ntiles * kOmega * 4; break;
+    case 3: src = p->kernel == SPX_KERNEL_TILE ? p->d_col : nullptr; bytes = p->ntiles * kOmega * p->sigma * 4; break;
+    case 4: src = p->kernel == SPX_KERNEL_TILE ? p->d_val : nullptr; bytes = p->ntiles * kOmega * p->sigma * 8; break;
+    case 5: src = p->d_coords; bytes = p->ncoords * 8; break;
+    default: return -1;
+  }
+  if (!src) return 0;
+  if (dst && bytes > 0) {
+    if (what == 5) {
+      std::vector<Coord> c((size_t)p->ncoords);
+      if (hipMemcpy(c.data(), src, (size_t)bytes, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+      std::vector<int64_t> o;
+      for (auto &x : c) { o.push_back(x.row); o.push_back(x.nz); }
+      int64_t ob = (int64_t)o.size() * 8;
+      std::memcpy(dst, o.data(), (size_t)std::min(ob, cap));
+      return ob;
+    }
+    if (hipMemcpy(dst, src, (size_t)std::min(bytes, cap), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  }
+  return what == 5 ? bytes * 2 : bytes;
+}
+
+int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, double *y, int warmup, int reps,
+                  int64_t flush_bytes, double *out_ms) {
+  if (!p || reps <= 0 || !out_ms) return fail(SPX_EARG, "bad timing arguments");
+  // Every execute gets its own event pairs, so the timed executes run
+  // back-to-back on one stream with a single host wait at the end.
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::vector<hipEvent_t> ev(4 * (size_t)reps, nullptr);
+  int rc = SPX_OK;
+  for (auto &e : ev)
+    if (rc == SPX_OK && hipEventCreate(&e) != hipSuccess) rc = fail(SPX_EDEV, "hipEventCreate failed");
+  void *flush = nullptr;
+  if (rc == SPX_OK && flush_bytes > 0 && hipMalloc(&flush, (size_t)flush_bytes) != hipSuccess)
+    rc = fail(SPX_EMEM, "hipMalloc of the cache-flush buffer failed");
+  for (int i = 0; i < warmup && rc == SPX_OK; i++) rc = execute_impl(p, alpha, x, beta, y, st, nullptr, nullptr);
+  for (int i = 0; i < reps && rc == SPX_OK; i++) {
+    if (flush) (void)hipMemsetAsync(flush, i & 0xff, (size_t)flush_bytes, st);
+    (void)hipEventRecord(ev[4 * i], st);
+    rc = execute_impl(p, alpha, x, beta, y, st, ev[4 * i + 1], ev[4 * i + 2]);
+    (void)hipEventRecord(ev[4 * i + 3], st);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess && rc == SPX_OK) rc = fail(SPX_EDEV, "stream synchronize failed");
+  double tot = 0, ker = 0;
+  if (rc == SPX_OK) {
+    for (int i = 0; i < reps; i++) {
+      float t = 0, tk = 0;
+      (void)hipEventElapsedTime(&t, ev[4 * i], ev[4 * i + 3]);
+      (void)hipEventElapsedTime(&tk, ev[4 * i + 1], ev[4 * i + 2]);
+      tot += t;
+      ker += tk;
+    }
+  }
+  if (flush) (void)hipFree(flush);
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(st);
+  if (rc != SPX_OK) return rc;
+  out_ms[0] = tot / reps;
+  out_ms[1] = ker / reps;
+  return SPX_OK;
+}
+
+// ---- device memory helpers -------------------------------------------------------
+
+void *spx_malloc(int64_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, (size_t)(bytes > 0 ? bytes : 1)) != hipSuccess) {
+    g_last_error = "hipMalloc failed";
+    return nullptr;
+  }
+  return p;
+}
+int spx_free(void *p) {
+  HIP_TRY(hipFree(p));
+  return SPX_OK;
+}
+int spx_memcpy_h2d(void *dst, const void *src, int64_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return SPX_OK;
+}
+int spx_memcpy_d2h(void *dst, const void *src, int64_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return SPX_OK;
+}
+int spx_memcpy_d2d(void *dst, const void *src, int64_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice));
+  return SPX_OK;
+}
+int spx_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return SPX_OK;
+}
+int spx_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// spx_internal.h -- types shared by the gfx950 kernels and the host plan code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace spx {
+
+constexpr int kBlock = 256;   // threads per workgroup (4 wave64s)
+constexpr int kWave = 64;     // CDNA wavefront width
+constexpr int kOmega = 64;    // lanes per CSR5-style tile
+
+// One nnz-contiguous partition of the matrix (a stream's share in the v1
+// mode, a task in the v2 mode, a row block in the baseline mode).
+// [s, e] is the nonzero range (e inclusive, as SB/src/dspmv_mgpu_v1.cu:62-69),
+// r0/r1 the rows holding nonzeros s and e, sflag/eflag whether those rows are
+// split at the partition boundary (v1:77, v1:89).  The partition owns rows
+// [row_lo, row_hi): r0..r1 plus the empty rows up to the next partition.
+struct Part {
+  int64_t s, e;
+  int32_t r0, r1, sflag, eflag;
+  int32_t row_lo, row_hi;
+  int64_t units;      // workgroup work items of this partition
+  int64_t unit_off;   // first entry in the unit table
+  int64_t carry_off;  // carry slot of the head split row; unit u's slots follow
+                      // at carry_off + 1 + 2u (+1) (per tile for the tile kernel)
+  int64_t cslot1;     // carry slot of the tail split row
+  int64_t coord_off;  // merge path: unit boundary coordinates (units + 1)
+  int64_t ntiles;     // tiles: count
+  int64_t tile_off;   // tiles: first global tile index
+  int64_t tp_off;     // tiles: first tile pointer (ntiles + 1 per partition)
+};
+
+// A unit of work: (partition, unit index within the partition).
+struct Unit {
+  int32_t part, local;
+};
+
+// Merge-path coordinate of a unit boundary: absolute row, absolute nonzero.
+struct Coord {
+  int32_t row, nz;
+};
+
+struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
+  float hi, lo;
+};
+
+template <typename V, typename X>
+struct KArgs {
+  const Part *parts;
+  const Unit *units;
+  int64_t unit_begin, unit_count;
+  int *queue;  // non-null: persistent workgroups pop units from this counter
+  const int32_t *rowptr;
+  const int32_t *col;
+  const V *val;
+  const X *x;
+  double alpha, beta;
+  double *y;
+  int32_t *crow;
+  double *cval;
+  const Coord *coords;
+  const uint32_t *tile_ptr;
+  const uint32_t *tile_flags;
+  int sigma;
+};
+
+// ---- launchers (kernels.hip) ---------------------------------------------
+enum { PART_NNZ = 0, PART_TASKS = 1, PART_ROWS = 2 };
+
+// Device partitioner: fills parts[0..P) (s, e, r0, r1, flags, row_lo/hi).
+hipError_t launch_partition(const int32_t *rowptr, int m, int64_t nnz, int P,
+                            int kind, Part *parts, hipStream_t st);
+hipError_t launch_merge_coords(const int32_t *rowptr, const Part *parts,
+                               int nparts, int64_t total_coords, int tw,
+                               Coord *coords, hipStream_t st);
+hipError_t launch_tile_convert(const int32_t *rowptr, const int32_t *col,
+                               const double *val, const Part *hparts,
+                               const Part *dparts, int nparts, int sigma,
+                               uint32_t *tile_ptr, uint32_t *flags,
+                               int32_t *colt, double *valt, float *valt32,
+                               hipStream_t st);
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
+hipError_t launch_split_x(const double *x, XPair *xhl, int64_t n,
+                          hipStream_t st);
+hipError_t launch_to_f32(const double *v, float *o, int64_t n, hipStream_t st);
+hipError_t launch_empty_rows(const int32_t *rows, int64_t nrows, double beta,
+                             double *y, hipStream_t st);
+hipError_t launch_fixup(const int32_t *crow, const double *cval, int64_t nc,
+                        double alpha, double beta, double *y, hipStream_t st);
+hipError_t launch_reset(int *q, int n, hipStream_t st);
+hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag,
+                             hipStream_t st);
+
+// SpMV main kernels.  grid = number of workgroups (persistent when a.queue).
+hipError_t launch_vector_f64(const KArgs<double, double> &a, int lanes, int grid, hipStream_t st);
+hipError_t launch_vector_f32(const KArgs<float, XPair> &a, int lanes, int grid, hipStream_t st);
+hipError_t launch_merge_f64(const KArgs<double, double> &a, int items, int grid, hipStream_t st);
+hipError_t launch_merge_f32(const KArgs<float, XPair> &a, int items, int grid, hipStream_t st);
+hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st);
+hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st);
+
+}  // namespace spx

@@ -1,0 +1,143 @@
+"""CPU checks of the C-ABI boundary: libspx.so loads, exports every function
+include/*.h declares (extern "C" and the reference's C++-linkage names), and
+its host-side logic (row search, argument and memory-guard rejections) acts
+as the reference's does.  No compute call is made here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import spx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in {"if", "while", "for", "return", "sizeof"}))
+
+
+def exported():
+    out = os.popen(f"nm -D --defined-only {spx.LIB_PATH}").read().split("\n")
+    return {l.split()[-1] for l in out if " T " in l}
+
+
+def test_capi_header_symbols_exported():
+    names = header_functions(os.path.join(ROOT, "include", "spx_capi.h"))
+    assert len(names) >= 25
+    ex = exported()
+    missing = [n for n in names if n not in ex]
+    assert not missing, missing
+    assert set(names) == set(spx.CAPI), set(names) ^ set(spx.CAPI)
+
+
+def test_reference_header_symbols_exported():
+    names = header_functions(os.path.join(ROOT, "include", "spmv_kernel.h"))
+    assert set(names) == set(spx.CXX_SYMBOLS)
+    ex = exported()
+    for n in names:
+        assert spx.CXX_SYMBOLS[n] in ex, n
+
+
+def test_library_loads_without_gpu():
+    L = spx.lib()
+    assert L.spx_version().startswith(b"spx")
+    assert L.spx_device_count() >= 0
+
+
+def test_get_row_from_index_matches_reference_restatement():
+    rng = np.random.default_rng(7)
+    for trial in range(40):
+        m = int(rng.integers(1, 60))
+        lens = rng.integers(0, 4, m)
+        rp = np.zeros(m + 1, np.int64)
+        rp[1:] = np.cumsum(lens)
+        for k in range(int(rp[-1]) + 1):
+            assert spx.get_row_from_index(rp, k) == oracle.get_row_from_index(rp, k)
+    # through the C++-linkage symbol as well
+    f = getattr(spx.lib(), spx.CXX_SYMBOLS["get_row_from_index"])
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_void_p, C.c_longlong]
+    rp = np.array([0, 5, 5, 10], np.int64)
+    assert f(3, rp.ctypes.data, 5) == 1
+
+
+def tiny():
+    rp = np.array([0, 2, 3], np.int64)
+    return 2, 2, 3, rp, np.array([0, 1, 1], np.int32), np.array([1.0, 2.0, 3.0]), np.ones(2), np.zeros(2)
+
+
+def test_dropin_rejects_ngpu_other_than_one():
+    m, n, nnz, rp, c, v, x, y = tiny()
+    for ng in (0, 2, 8):
+        assert spx.spMV_mgpu_baseline(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, ngpu=ng)[0] == -1
+        assert spx.spMV_mgpu_v1(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, ngpu=ng, kernel=1)[0] == -1
+        assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, ngpu=ng, kernel=1, nb=1, q=1)[0] == -1
+
+
+def test_dropin_rejects_bad_kernel_and_v2_args():
+    m, n, nnz, rp, c, v, x, y = tiny()
+    for k in (0, 4, -1):
+        assert spx.spMV_mgpu_v1(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=k)[0] == -1
+        assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=k, nb=1, q=1)[0] == -1
+    # SB/src/dspmv_mgpu_v2.cu:44-46
+    assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=1, nb=0, q=1)[0] == -1
+    assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=1, nb=1, q=0)[0] == -1
+
+
+def test_memory_guard_rejects_through_the_seam():
+    # footprint > 0.8 * free device memory -> -1 (SB/src/dspmv_mgpu_v1.cu:106-117)
+    m, n, nnz, rp, c, v, x, y = tiny()
+    L = spx.lib()
+    try:
+        L.spx_debug_set_free_mem(10)  # 10 bytes "free"
+        assert L.spx_get_gpu_availble_mem(1) == pytest.approx(1e-8)
+        assert spx.spMV_mgpu_baseline(m, n, nnz, 1.0, v, rp, c, x, 0.0, y)[0] == -1
+        for k in (1, 2, 3):
+            assert spx.spMV_mgpu_v1(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=k)[0] == -1
+        # v2 clamps nb to the memory cap; a zero cap is rejected
+        assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=1, nb=100, q=1)[0] == -1
+    finally:
+        L.spx_debug_set_free_mem(-1)
+
+
+def desc(**kw):
+    d = spx.PlanDesc()
+    spx.lib().spx_plan_desc_init(C.byref(d))
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def test_plan_create_validates_before_touching_the_device():
+    L = spx.lib()
+    rp = np.array([0, 2, 3], np.int64)
+    bad_rp = np.array([0, 3, 2], np.int64)
+    c = np.array([0, 1, 1], np.int32)
+    v = np.array([1.0, 2.0, 3.0])
+    h = C.c_void_p()
+    base = dict(m=2, n=2, nnz=3, rowptr64=rp.ctypes.data, col=c.ctypes.data, val=v.ctypes.data)
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, kernel=4))) == spx.EARG
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, precision=7))) == spx.EARG
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**dict(base, rowptr64=bad_rp.ctypes.data)))) == spx.EARG
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**dict(base, nnz=4)))) == spx.EARG
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**dict(base, nnz=2 ** 31)))) == spx.ERANGE
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, mode=spx.MODE_QUEUE, nb=0))) == spx.EARG
+    assert b"nb" in L.spx_last_error()
+    assert L.spx_plan_create(C.byref(h), None) == spx.EARG
+
+
+def test_product_package_has_no_oracle_dependency():
+    # the product path must never import or link the oracle
+    pkg = os.path.join(ROOT, "sparse-blas_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "liboracle" not in text and "oracle.h" not in text, f

@@ -1,0 +1,309 @@
+"""GPU parity: libspx.so on the MI355X against the CPU oracle.
+
+Bar (FP64 path): normwise relative error <= 1e-13 against the oracle AND every
+row within 2*gamma_k*|alpha|(|A||x|)_i + 2u|beta y0_i| (k = row length,
+gamma_k = k u / (1 - k u), u = 2^-53: the oracle and the kernel each err by at
+most gamma_k); bit-exact on small-integer inputs, where every partial sum is
+an exact integer so any summation order gives the oracle's bits.
+Precision form: normwise error <= 1e-7 (stated per matrix).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import spx
+from spx import workloads as W
+
+pytestmark = pytest.mark.gpu
+U = 2.0 ** -53
+
+
+def run_plan(A, x, alpha=1.0, beta=0.0, y0=None, **kw):
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], **kw)
+    y = P(x, alpha, beta, y0)
+    return y, P
+
+
+def check_fp64(A, x, y, alpha, beta, y0, tol=1e-13):
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, alpha, beta, y0)
+    _, mag = oracle.spmv_dot2(A["rowptr"], A["col"], A["val"], x, alpha, beta, y0)
+    k = np.diff(A["rowptr"]).astype(float)
+    gam = k * U / (1 - k * U)
+    yb = np.abs(beta * y0) if (y0 is not None and beta != 0) else 0.0
+    bound = 2 * gam * mag + 2 * U * yb + 1e-300
+    err = np.abs(y - ref)
+    bad = np.nonzero(err > bound)[0]
+    assert bad.size == 0, f"{bad.size} rows over the bound, first {bad[:5]} err {err[bad[:5]]} bound {bound[bad[:5]]}"
+    nrm = np.linalg.norm(ref)
+    rel = np.linalg.norm(y - ref) / nrm if nrm > 0 else np.linalg.norm(y)
+    assert rel <= tol, rel
+    return rel
+
+
+KERNEL_CASES = [
+    dict(kernel=1), dict(kernel=1, lanes=64), dict(kernel=1, lanes=2),
+    dict(kernel=2), dict(kernel=2, items=2), dict(kernel=2, items=16),
+    dict(kernel=3), dict(kernel=3, sigma=4), dict(kernel=3, sigma=32), dict(kernel=3, sigma=7),
+]
+MODE_CASES = [
+    dict(mode=spx.MODE_NNZ, parts=1), dict(mode=spx.MODE_NNZ, parts=3), dict(mode=spx.MODE_NNZ, parts=8),
+    dict(mode=spx.MODE_ROWS, parts=1), dict(mode=spx.MODE_ROWS, parts=4),
+    dict(mode=spx.MODE_QUEUE, nb=5000, queues=1), dict(mode=spx.MODE_QUEUE, nb=777, queues=3),
+]
+
+
+@pytest.fixture(scope="module")
+def c1():
+    A = W.zipf(8192, 1.5, 512, seed=1)
+    x = W.vector(A["n"], 101)
+    return A, x
+
+
+@pytest.mark.parametrize("kc", KERNEL_CASES, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+@pytest.mark.parametrize("mc", MODE_CASES, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_c1_all_kernels_and_modes(c1, kc, mc):
+    A, x = c1
+    print("case", kc, mc, flush=True)
+    y, _ = run_plan(A, x, 1.0, 0.0, None, **kc, **mc)
+    check_fp64(A, x, y, 1.0, 0.0, None)
+    y0 = W.vector(A["m"], 7)
+    y, _ = run_plan(A, x, 0.7, -0.3, y0, **kc, **mc)
+    check_fp64(A, x, y, 0.7, -0.3, y0)
+
+
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("mc", MODE_CASES, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_small_integers_bit_exact(kernel, mc):
+    A = W.small_int(3000, 2000, 12, seed=5)
+    x = W.vector(A["n"], 9, W.VAL_SMALLINT)
+    y0 = W.vector(A["m"], 10, W.VAL_SMALLINT)
+    y, _ = run_plan(A, x, 2.0, -3.0, y0, kernel=kernel, **mc)
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 2.0, -3.0, y0)
+    assert np.array_equal(y, ref)
+
+
+def edge_matrices():
+    rng = np.random.default_rng(3)
+    out = {}
+
+    def mk(lens, n, name):
+        lens = np.asarray(lens, np.int64)
+        rp = np.zeros(len(lens) + 1, np.int64)
+        rp[1:] = np.cumsum(lens)
+        nnz = int(rp[-1])
+        col = np.concatenate([np.sort(rng.choice(n, int(k), replace=False)) for k in lens]).astype(np.int32) \
+            if nnz else np.zeros(0, np.int32)
+        val = rng.uniform(-1, 1, nnz)
+        out[name] = W.finish(rp, col, val, len(lens), n)
+
+    mk([0, 0, 0], 5, "all_empty")
+    mk([3], 3, "one_row")
+    mk([1], 1, "one_by_one")
+    mk([0, 0, 5, 0, 0, 0, 7, 0, 0], 10, "empty_runs")
+    lens = np.zeros(500, np.int64)
+    lens[250] = 20000
+    mk(lens, 30000, "one_long_row_among_empty")
+    lens = rng.integers(0, 3, 5000)
+    lens[::97] = 0
+    mk(lens, 4000, "short_with_empties")
+    mk(np.full(4096, 16), 4096, "aligned_tiles")  # row ends on tile boundaries
+    lens = rng.integers(1, 5, 2000)
+    lens[[5, 900, 1999]] = [9000, 15000, 12000]
+    mk(lens, 20000, "long_rows_at_edges")
+    return out
+
+
+EDGE = edge_matrices()
+
+
+@pytest.mark.parametrize("name", list(EDGE))
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("parts", [1, 2, 5])
+def test_edge_cases(name, kernel, parts):
+    A = EDGE[name]
+    x = W.vector(max(A["n"], 1), 11)[: A["n"]]
+    y0 = np.full(A["m"], np.nan)  # beta = 0 must not read y
+    y, _ = run_plan(A, x, 1.25, 0.0, y0, kernel=kernel, parts=parts)
+    assert np.all(np.isfinite(y))
+    check_fp64(A, x, y, 1.25, 0.0, None)
+    y0 = W.vector(A["m"], 12)
+    y, _ = run_plan(A, x, -0.5, 2.0, y0, kernel=kernel, parts=parts, mode=spx.MODE_QUEUE, nb=997, queues=2)
+    check_fp64(A, x, y, -0.5, 2.0, y0)
+
+
+def test_partitions_bit_exact_against_host_restatement():
+    A = W.zipf(5000, 1.3, 900, seed=8)
+    for P in (1, 2, 3, 7, 16):
+        _, plan = run_plan(A, W.vector(A["n"], 1), kernel=2, parts=P)
+        got = plan.export(0).reshape(-1, 6)
+        ref = oracle.v1_partition(A["rowptr"], P, rule=1)
+        exp = np.stack([ref["start_idx"], ref["end_idx"], ref["start_row"], ref["end_row"],
+                        ref["start_flag"], ref["end_flag"]], 1).astype(np.int64)
+        assert np.array_equal(got, exp), P
+    for nb in (1000, 4321, 99999):
+        _, plan = run_plan(A, W.vector(A["n"], 1), kernel=2, mode=spx.MODE_QUEUE, nb=nb)
+        got = plan.export(0).reshape(-1, 6)
+        ref = oracle.v2_tasks(A["rowptr"], nb, rule=1)
+        exp = np.stack([ref["start_idx"], ref["end_idx"], ref["start_row"], ref["end_row"],
+                        ref["start_flag"], ref["end_flag"]], 1).astype(np.int64)
+        assert np.array_equal(got, exp), nb
+
+
+def local_csr(A, part_rows, p):
+    """Partition p as its own CSR (SB/src/dspmv_mgpu_v1.cu:125-133, split rows clamped)."""
+    s, e, r0, r1, sf, ef = part_rows[p]
+    P = len(part_rows)
+    row_lo = 0 if p == 0 else r0
+    if p == P - 1:
+        row_hi = A["m"]
+    elif ef:
+        row_hi = r1 + 1
+    else:
+        row_hi = part_rows[p + 1][2]
+    nz = e - s + 1
+    lrp = np.clip(A["rowptr"][row_lo:row_hi + 1] - s, 0, nz)
+    return lrp, A["col"][s:e + 1], A["val"][s:e + 1]
+
+
+@pytest.mark.parametrize("sigma", [0, 4, 9, 32])
+@pytest.mark.parametrize("parts", [1, 3])
+def test_tile_descriptors_bit_exact(sigma, parts):
+    A = EDGE["short_with_empties"] if sigma == 9 else W.zipf(6000, 1.4, 700, seed=4)
+    _, plan = run_plan(A, W.vector(A["n"], 1), kernel=3, sigma=sigma, parts=parts)
+    info = plan.info()
+    sg = info["sigma"]
+    if sigma == 0:
+        assert sg == oracle.tile_sigma(A["m"], A["nnz"])
+    rows = plan.export(0).reshape(-1, 6)
+    tp, fl, ct, vt = plan.export(1), plan.export(2), plan.export(3), plan.export(4)
+    otp, ofl, oct_, ovt = [], [], [], []
+    for p in range(len(rows)):
+        lrp, c, v = local_csr(A, rows, p)
+        a, b, cc, vv = oracle.tile_convert(lrp, c, v, sg)
+        otp.append(a); ofl.append(b); oct_.append(cc); ovt.append(vv)
+    assert np.array_equal(tp, np.concatenate(otp))
+    assert np.array_equal(fl, np.concatenate(ofl))
+    assert np.array_equal(ct, np.concatenate(oct_))
+    assert np.array_equal(vt, np.concatenate(ovt))
+
+
+def test_merge_coordinates_on_the_merge_path():
+    A = W.zipf(4000, 1.5, 512, seed=6)
+    _, plan = run_plan(A, W.vector(A["n"], 1), kernel=2, items=4)
+    co = plan.export(5).reshape(-1, 2)
+    rp = A["rowptr"]
+    tw = 256 * 4
+    L = A["m"] + A["nnz"]
+    for u, (i, j) in enumerate(co):
+        d = min(u * tw, L)
+        assert i + j == d
+        # consumed row ends precede nonzero j; unconsumed ones follow nonzero j-1
+        assert i == 0 or rp[i] <= j
+        assert i == A["m"] or j == 0 or rp[i + 1] > j - 1
+
+
+def test_reproducible_bits():
+    A = W.zipf(8192, 1.2, 2000, seed=2)
+    x = W.vector(A["n"], 3)
+    for kc in ({"kernel": 2}, {"kernel": 3}, {"kernel": 1, "parts": 4}):
+        y1, _ = run_plan(A, x, **kc)
+        y2, _ = run_plan(A, x, **kc)
+        assert np.array_equal(y1, y2)
+
+
+# ---- the reference entry points -----------------------------------------------------
+
+def g_inputs(n):
+    A = W.g_matrix(n)
+    alpha, beta = oracle.harness_alpha_beta(1, A["nnz"])  # draws right after the values
+    return A, np.ones(n), alpha, beta
+
+
+@pytest.mark.parametrize("n", [200, 1000])
+def test_dropin_g_matrix_all_entry_points(n):
+    A, x, alpha, beta = g_inputs(n)
+    m, nnz = A["m"], A["nnz"]
+    y0 = np.zeros(m)
+    args = (m, n, nnz, alpha, A["val"], A["rowptr"], A["col"], x, beta, y0)
+    rc, y = spx.spMV_mgpu_baseline(*args)
+    assert rc == 0
+    check_fp64(A, x, y, alpha, beta, y0)
+    for k in (1, 2, 3):
+        rc, y = spx.spMV_mgpu_v1(*args, ngpu=1, kernel=k)
+        assert rc == 0
+        check_fp64(A, x, y, alpha, beta, y0)
+        # the harness's v2 search: nb = nnz/(d*c), q = c (dspmv_test.cu:314-332)
+        for c in (1, 2, 4, 8):
+            rc, y = spx.spMV_mgpu_v2(*args, ngpu=1, kernel=k, nb=nnz // c, q=c)
+            assert rc == 0
+            check_fp64(A, x, y, alpha, beta, y0)
+
+
+def test_dropin_nonzero_beta_and_v1_streams():
+    A = W.zipf(3000, 1.5, 512, seed=11)
+    x = W.vector(A["n"], 2)
+    y0 = W.vector(A["m"], 3)
+    old = spx.set_v1_streams(4)
+    try:
+        for k in (1, 2, 3):
+            rc, y = spx.spMV_mgpu_v1(A["m"], A["n"], A["nnz"], 0.9, A["val"], A["rowptr"], A["col"], x, 0.4, y0,
+                                     kernel=k)
+            assert rc == 0
+            check_fp64(A, x, y, 0.9, 0.4, y0)
+    finally:
+        spx.set_v1_streams(old)
+
+
+def test_csr5_kernel_device_pointers():
+    A = W.zipf(4000, 1.5, 512, seed=12)
+    x = W.vector(A["n"], 4)
+    y0 = W.vector(A["m"], 5)
+    rp32 = A["rowptr"].astype(np.int32)
+    bufs = [spx.DeviceBuffer.from_array(a) for a in (A["val"], rp32, A["col"], x, y0)]
+    rc = spx.csr5_kernel(A["m"], A["n"], A["nnz"], 1.5, *bufs[:4], -0.5, bufs[4])
+    assert rc == 0
+    y = bufs[4].download(np.float64, A["m"])
+    check_fp64(A, x, y, 1.5, -0.5, y0)
+
+
+# ---- full-size workloads --------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c2():
+    A = W.circuit()
+    x = W.vector(A["n"], 22)
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    return A, x, ref
+
+
+@pytest.mark.parametrize("kc", [dict(kernel=2), dict(kernel=3), dict(kernel=2, parts=8),
+                                dict(kernel=2, mode=spx.MODE_QUEUE, nb=59_500_000 // 8, queues=8)])
+def test_c2_circuit_full_size(c2, kc):
+    A, x, ref = c2
+    print("c2 case", kc, flush=True)
+    y, _ = run_plan(A, x, **kc)
+    rel = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    assert rel <= 1e-13, rel
+    # per-row bound on a sample of rows (the compensated reference is slower)
+    k = np.diff(A["rowptr"]).astype(float)
+    big = np.argsort(k)[-50:]
+    for r in big:
+        lo, hi = A["rowptr"][r], A["rowptr"][r + 1]
+        prods = A["val"][lo:hi] * x[A["col"][lo:hi]]
+        mag = np.abs(prods).sum()
+        assert abs(y[r] - ref[r]) <= 2 * k[r] * U / (1 - k[r] * U) * mag
+
+
+@pytest.mark.parametrize("name", ["rail", "asic"])
+@pytest.mark.parametrize("kernel", [2, 3])
+def test_precision_form(name, kernel):
+    A = W.rail() if name == "rail" else W.asic()
+    x = W.vector(A["n"], 31)
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    y, _ = run_plan(A, x, kernel=kernel, precision=spx.PREC_F32)
+    rel = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    print(f"precision form {name} k{kernel}: normwise rel err {rel:.3e}", flush=True)
+    assert rel <= 1e-7
+    if name == "rail":
+        # values are exact in FP32; x as a (hi, lo) pair keeps ~48 bits
+        assert rel <= 1e-13
