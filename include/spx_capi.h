@@ -70,6 +70,9 @@ typedef struct spx_plan_desc {
   int items;                /* MERGE: merge items per thread, 1..16 (0 = auto) */
   int precision;            /* SPX_PREC_* */
   int flags;                /* SPX_FLAG_* bits */
+  int64_t long_row;         /* rows with at least this many nonzeros take the
+                               column-blocked long-row path (x staged in LDS);
+                               0 = automatic, < 0 = never */
 } spx_plan_desc;
 
 /* desc.flags: the row pointer, column and value arrays are DEVICE memory
@@ -102,7 +105,9 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
 /* Plan facts: writes up to `cap` int64 values:
  * [0] m [1] n [2] nnz [3] kernel [4] mode [5] parts [6] sigma [7] lanes
  * [8] items [9] units (workgroup work items) [10] carries [11] tiles
- * [12] device bytes held [13] precision [14] queues. Returns the count. */
+ * [12] device bytes held [13] precision [14] queues [15] long rows
+ * [16] long-row nonzeros [17] column blocks [18] carry runs
+ * [19] long-row threshold. Returns the count. */
 int spx_plan_info(spx_plan_t p, int64_t *out, int cap);
 
 /* Copy plan internals to host for bit-exact checks.  `what`:

@@ -87,17 +87,42 @@ __device__ inline void put_carry(int32_t *crow, double *cval, int64_t slot, int 
   cval[slot] = v;
 }
 
-// Next unit for this workgroup: blockIdx-strided, or popped from the device
-// work queue by persistent workgroups.  Uniform across the workgroup.
-__device__ inline int64_t next_unit(int *queue, int64_t iter, int *s_slot) {
-  if (queue) {
-    if (threadIdx.x == 0) *s_slot = atomicAdd(queue, 1);
-    __syncthreads();
-    int u = *s_slot;
-    __syncthreads();
-    return u;
+// The dispatcher hands workgroup b to XCD b % kXcd.  Units are numbered in
+// nonzero order, so giving each XCD one contiguous range of units keeps the
+// x entries a unit gathers in that XCD's own L2 (neighbouring units touch
+// neighbouring columns for banded rows).
+__device__ inline int64_t xcd_unit(int64_t b, int64_t count) {
+  const int64_t per = (count + kXcd - 1) / kXcd;
+  return (b % kXcd) * per + b / kXcd;
+}
+
+// Next unit for this workgroup, uniform across the workgroup.  Direct
+// launches (grid = kXcd * ceil(count / kXcd)) take one unit each.  Persistent
+// workgroups pop units from a device work queue: one counter per XCD over
+// that XCD's contiguous share, and when the own share is drained they steal
+// from the other XCDs' shares in turn.  Returns >= count when done.
+__device__ inline int64_t next_unit(int *queue, int64_t iter, int64_t count, int *s_slot) {
+  if (!queue) return iter == 0 ? xcd_unit(blockIdx.x, count) : count;
+  if (threadIdx.x == 0) {
+    const int64_t per = (count + kXcd - 1) / kXcd;
+    const int home = blockIdx.x % kXcd;
+    int64_t u = count;
+    for (int k = 0; k < kXcd; k++) {
+      const int x = (home + k) % kXcd;
+      const int64_t lim = per * (x + 1) < count ? per * (x + 1) : count;
+      if ((int64_t)per * x >= lim) continue;
+      const int64_t got = per * x + atomicAdd(&queue[x], 1);
+      if (got < lim) {
+        u = got;
+        break;
+      }
+    }
+    *s_slot = (int)(u < count ? u : count);
   }
-  return (int64_t)blockIdx.x + iter * gridDim.x;
+  __syncthreads();
+  const int64_t u = *s_slot;
+  __syncthreads();
+  return u;
 }
 
 // ---- partitioner ---------------------------------------------------------------
@@ -303,25 +328,91 @@ __global__ void k_reset(int *q, int n) {
   if (i < n) q[i] = 0;
 }
 
-// Sum each carried row's slots in slot order.  Slots of one row are contiguous
-// apart from empty (-1) slots, so the first valid slot of a row sums the run.
-__global__ void k_fixup(const int32_t *__restrict__ crow, const double *__restrict__ cval, int64_t nc,
-                        double alpha, double beta, double *__restrict__ y) {
+// Sum each carried row's slots in slot order.  Which slots hold which row is
+// fixed by the sparsity structure alone, so the plan records the runs once
+// (row, first slot, end slot) and one thread per run does the sum.  Slots
+// inside a run that belong to no row (-1) are skipped.
+__global__ void k_fixup(const Run *__restrict__ runs, int64_t nruns, const int32_t *__restrict__ crow,
+                        const double *__restrict__ cval, double alpha, double beta, double *__restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
-  int r = crow[i];
-  if (r < 0) return;
-  int64_t k = i - 1;
-  while (k >= 0 && crow[k] < 0) k--;
-  if (k >= 0 && crow[k] == r) return;
+  if (i >= nruns) return;
+  const Run R = runs[i];
   double s = 0.0;
-  for (k = i; k < nc; k++) {
-    int rr = crow[k];
-    if (rr < 0) continue;
-    if (rr != r) break;
-    s += cval[k];
+  for (int64_t k = R.beg; k < R.end; k++)
+    if (crow[k] == R.row) s += cval[k];
+  y[R.row] = y_update(alpha, beta, s, y, R.row);
+}
+
+// ---- long rows: column blocks with x staged in LDS -----------------------------------
+// Rows with very many nonzeros at scattered columns make every x gather a
+// cache miss.  Those rows are kept apart (columns sorted per row) and swept
+// block by block over the columns: a workgroup stages x[c0, c0 + kColBlock)
+// in LDS with coalesced loads, then each wave takes a long row's slice of
+// nonzeros inside that column block (found by binary search at plan time)
+// and reduces it.  Per-(block, row) partials are summed per row in block
+// order by k_long_reduce, so the result is deterministic.
+
+__global__ void k_long_seg(const int32_t *__restrict__ lrowptr, const int32_t *__restrict__ lcol, int nlong,
+                           int nblocks, int32_t *__restrict__ seg) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)(nblocks + 1) * nlong) return;
+  const int b = (int)(g / nlong), r = (int)(g % nlong);
+  int lo = lrowptr[r], hi = lrowptr[r + 1];
+  const int64_t key = (int64_t)b * kColBlock;
+  while (lo < hi) {
+    int mid = lo + ((hi - lo) >> 1);
+    if ((int64_t)lcol[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  y[r] = y_update(alpha, beta, s, y, r);
+  seg[g] = lo;
+}
+
+__device__ inline double prod_lds(double v, const double *sx, int c) { return v * sx[c]; }
+__device__ inline double prod_lds(float v, const XPair *sx, int c) { return prod(v, sx, c); }
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kBlock) void k_long(const int32_t *__restrict__ seg, int nlong, int nblocks,
+                                                 const int32_t *__restrict__ lcol, const V *__restrict__ lval,
+                                                 const X *__restrict__ x, int n, double *__restrict__ partial) {
+  __shared__ X s_x[kColBlock];
+  const int64_t b = xcd_unit(blockIdx.x, nblocks);
+  if (b >= nblocks) return;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const int64_t c0 = b * kColBlock;
+  const int cn = (int)((int64_t)n - c0 < kColBlock ? (int64_t)n - c0 : kColBlock);
+  for (int k = threadIdx.x; k < cn; k += kBlock) s_x[k] = x[c0 + k];
+  __syncthreads();
+  for (int r = wid; r < nlong; r += kBlock / kWave) {
+    const int s = seg[b * nlong + r], e = seg[(b + 1) * nlong + r];
+    double sum = 0.0, sum2 = 0.0;
+    int j = s + lane;
+    for (; j + kWave < e; j += 2 * kWave) {
+      sum += prod_lds(lval[j], s_x, (int)(lcol[j] - c0));
+      sum2 += prod_lds(lval[j + kWave], s_x, (int)(lcol[j + kWave] - c0));
+    }
+    if (j < e) sum += prod_lds(lval[j], s_x, (int)(lcol[j] - c0));
+    sum += sum2;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kWave);
+    if (lane == 0) partial[b * nlong + r] = sum;
+  }
+}
+
+// One wave per long row: y[row] += alpha * sum over blocks (the regular pass
+// has already written beta*y0 for the row, which is empty there).
+__global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong, int nblocks,
+                                                        const int32_t *__restrict__ rows, double alpha,
+                                                        double *__restrict__ y) {
+  const int r = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int lane = threadIdx.x % kWave;
+  if (r >= nlong) return;
+  double s = 0.0;
+  for (int b = lane; b < nblocks; b += kWave) s += partial[(int64_t)b * nlong + r];
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (lane == 0) y[rows[r]] += alpha * s;
 }
 
 // ---- kernel 1: CSR-vector, L lanes per row --------------------------------------
@@ -335,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V, X> a) {
   __shared__ int s_slot;
   const int g = threadIdx.x / L, l = threadIdx.x % L;
   for (int64_t iter = 0;; iter++) {
-    int64_t u = next_unit(a.queue, iter, &s_slot);
+    int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
     if (u >= a.unit_count) break;
     const Unit U = a.units[a.unit_begin + u];
     const Part &q = a.parts[U.part];
@@ -387,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V, X> a) {
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
 
   for (int64_t iter = 0;; iter++) {
-    int64_t u = next_unit(a.queue, iter, &s_slot);
+    int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
     if (u >= a.unit_count) break;
     const Unit U = a.units[a.unit_begin + u];
     const Part &q = a.parts[U.part];
@@ -528,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   const int sigma = a.sigma;
   for (int64_t iter = 0;; iter++) {
-    int64_t u = next_unit(a.queue, iter, &s_slot);
+    int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
     if (u >= a.unit_count) break;
     const Unit U = a.units[a.unit_begin + u];
     const Part &q = a.parts[U.part];
@@ -694,10 +785,39 @@ hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag, hipStr
   return hipGetLastError();
 }
 
-hipError_t launch_fixup(const int32_t *crow, const double *cval, int64_t nc, double alpha, double beta,
-                        double *y, hipStream_t st) {
-  if (nc <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fixup, dim3(nblk(nc)), dim3(kBlock), 0, st, crow, cval, nc, alpha, beta, y);
+hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow, const double *cval, double alpha,
+                        double beta, double *y, hipStream_t st) {
+  if (nruns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fixup, dim3(nblk(nruns)), dim3(kBlock), 0, st, runs, nruns, crow, cval, alpha, beta, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_long_seg(const int32_t *lrowptr, const int32_t *lcol, int nlong, int nblocks, int32_t *seg,
+                           hipStream_t st) {
+  const int64_t tot = (int64_t)(nblocks + 1) * nlong;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_long_seg, dim3(nblk(tot)), dim3(kBlock), 0, st, lrowptr, lcol, nlong, nblocks, seg);
+  return hipGetLastError();
+}
+
+hipError_t launch_long_f64(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol, const double *lval,
+                           const double *x, int n, double *partial, hipStream_t st) {
+  hipLaunchKernelGGL((k_long<double, double>), dim3(direct_grid(nblocks)), dim3(kBlock), 0, st, seg, nlong, nblocks,
+                     lcol, lval, x, n, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_long_f32(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol, const float *lval,
+                           const XPair *x, int n, double *partial, hipStream_t st) {
+  hipLaunchKernelGGL((k_long<float, XPair>), dim3(direct_grid(nblocks)), dim3(kBlock), 0, st, seg, nlong, nblocks,
+                     lcol, lval, x, n, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_long_reduce(const double *partial, int nlong, int nblocks, const int32_t *rows, double alpha,
+                              double *y, hipStream_t st) {
+  hipLaunchKernelGGL(k_long_reduce, dim3(nblk((int64_t)nlong * kWave)), dim3(kBlock), 0, st, partial, nlong, nblocks,
+                     rows, alpha, y);
   return hipGetLastError();
 }
 

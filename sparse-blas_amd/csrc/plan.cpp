@@ -1,19 +1,25 @@
 // plan.cpp -- device-resident SpMV plans behind the extern "C" ABI.
 //
-// spx_plan_create uploads A once, partitions it on the device (nnz-balanced
-// partitions -> HIP streams, each partition -> workgroup units) and runs the
-// kernel-specific conversion (merge-path unit coordinates, or CSR5-style
-// tiles).  spx_plan_execute then only launches the SpMV kernels and the
-// deterministic carry fix-up.
+// spx_plan_create uploads A once and prepares everything the kernels need:
+//  - rows with very many nonzeros ("long rows") are split off into their own
+//    CSR, swept by column blocks with x staged in LDS (k_long);
+//  - the remaining "regular" CSR is partitioned on the device (nnz-balanced
+//    partitions -> HIP streams; each partition -> workgroup units);
+//  - kernel-specific conversion: merge-path unit coordinates, or CSR5-style
+//    tiles;
+//  - a dry execute records which carry slots belong to which split row (a
+//    property of the sparsity structure), giving the fix-up its run table.
+// spx_plan_execute then only launches kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/spx_capi.h"
@@ -61,12 +67,12 @@ int tile_sigma_auto(int m, int64_t nnz) {
 
 struct spx_plan {
   int m = 0, n = 0;
-  int64_t nnz = 0;
-  int kernel = 2, mode = SPX_MODE_NNZ, nparts = 1, queues = 1;
+  int64_t nnz = 0;      // whole matrix
+  int64_t nnz_reg = 0;  // regular part
+  int kernel = 2, mode = SPX_MODE_NNZ, queues = 1;
   int sigma = 0, lanes = 0, items = 0, precision = SPX_PREC_F64;
-  int64_t nb = 0;
 
-  int32_t *d_rowptr = nullptr;
+  int32_t *d_rowptr = nullptr;   // regular CSR row pointer (long rows empty)
   int32_t *d_col = nullptr;      // CSR columns, or tile layout for kernel 3
   double *d_val = nullptr;       // FP64 values (CSR or tile layout)
   float *d_val32 = nullptr;      // FP32 values (precision form)
@@ -82,10 +88,20 @@ struct spx_plan {
   int32_t *d_crow = nullptr;
   double *d_cval = nullptr;
   int64_t ncarry = 0;
-  bool any_carry = false;
+  Run *d_runs = nullptr;
+  int64_t nruns = 0;
   int32_t *d_empty = nullptr;
   int64_t nempty = 0;
   int *d_queue = nullptr;
+
+  // long rows
+  int nlong = 0, nblocks = 0;
+  int64_t long_nnz = 0, long_th = 0;
+  int32_t *d_lrowptr = nullptr, *d_lcol = nullptr, *d_lrows = nullptr, *d_seg = nullptr;
+  double *d_lval = nullptr, *d_partial = nullptr;
+  float *d_lval32 = nullptr;
+  hipStream_t lstream = nullptr;
+  hipEvent_t lfork = nullptr, ldone = nullptr;
 
   struct Launch {
     int stream;
@@ -101,11 +117,16 @@ struct spx_plan {
   ~spx_plan() {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
-                    (void *)d_crow, (void *)d_cval, (void *)d_empty, (void *)d_queue})
+                    (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
+                    (void *)d_lrowptr, (void *)d_lcol, (void *)d_lrows, (void *)d_seg, (void *)d_lval,
+                    (void *)d_partial, (void *)d_lval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
     for (auto e : done) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
+    if (lstream) (void)hipStreamDestroy(lstream);
+    if (lfork) (void)hipEventDestroy(lfork);
+    if (ldone) (void)hipEventDestroy(ldone);
   }
 
   template <typename T>
@@ -120,7 +141,18 @@ struct spx_plan {
     bytes += (int64_t)sizeof(T) * count;
     return SPX_OK;
   }
+
+  template <typename T>
+  int upload(T **p, const T *src, int64_t count) {
+    int rc = alloc(p, count);
+    if (rc) return rc;
+    if (count > 0) HIP_TRY(hipMemcpy(*p, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice));
+    return SPX_OK;
+  }
 };
+
+static int execute_impl(spx_plan *P, double alpha, const double *x, double beta, double *y, hipStream_t st,
+                        hipEvent_t t0, hipEvent_t t1, bool dry);
 
 extern "C" {
 
@@ -133,7 +165,19 @@ void spx_plan_desc_init(spx_plan_desc *d) {
 }
 
 const char *spx_last_error(void) { return g_last_error.c_str(); }
-const char *spx_version(void) { return "spx 0.1 gfx950"; }
+const char *spx_version(void) { return "spx 0.2 gfx950"; }
+
+}  // extern "C"
+
+// Long-row threshold: rows at least this long are swept by column blocks.
+// A row is worth it when it averages at least a few nonzeros per column
+// block; below 4096 nonzeros the per-block overhead dominates.
+static int64_t long_threshold(const spx_plan_desc *d) {
+  if (d->long_row < 0) return -1;
+  if (d->long_row > 0) return d->long_row;
+  const int64_t blocks = ((int64_t)d->n + kColBlock - 1) / kColBlock;
+  return std::max<int64_t>(4096, 4 * blocks);
+}
 
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int m = d->m, n = d->n;
@@ -145,6 +189,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   if (d->precision != SPX_PREC_F64 && d->precision != SPX_PREC_F32) return fail(SPX_EARG, "bad precision");
   if (m > 0 && !d->rowptr64 && !d->rowptr32) return fail(SPX_EARG, "missing row pointer");
   if (nnz > 0 && (!d->col || !d->val)) return fail(SPX_EARG, "missing col/val");
+  if (d->mode == SPX_MODE_QUEUE && d->nb <= 0) return fail(SPX_EARG, "nb must be positive");
   P->m = m;
   P->n = n;
   P->nnz = nnz;
@@ -152,10 +197,9 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   P->mode = d->mode;
   P->precision = d->precision;
   P->queues = d->queues > 0 ? d->queues : 1;
-  P->nb = d->nb;
   if (m == 0) return SPX_OK;
 
-  // host int32 row pointer (validated: monotone, ends at nnz)
+  // ---- host row pointer (validated: monotone, from 0 to nnz) ----
   const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
   std::vector<int64_t> rp64;
   std::vector<int32_t> rp32;
@@ -180,7 +224,78 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   }
   if (rp[0] != 0 || rp[m] != nnz) return fail(SPX_EARG, "row pointer must start at 0 and end at nnz");
 
-  // partitions
+  // ---- split off long rows ----
+  P->long_th = long_threshold(d);
+  std::vector<int32_t> long_rows;
+  if (P->long_th > 0)
+    for (int i = 0; i < m; i++)
+      if (rp[i + 1] - rp[i] >= P->long_th) long_rows.push_back(i);
+  P->nlong = (int)long_rows.size();
+
+  const int32_t *src_col = d->col;
+  const double *src_val = d->val;
+  bool src_dev = on_dev;
+  std::vector<int32_t> hcol, rcol, lcol, lrp;
+  std::vector<double> hval, rval, lval;
+  if (P->nlong > 0) {
+    if (on_dev) {
+      hcol.resize(nnz);
+      hval.resize(nnz);
+      HIP_TRY(hipMemcpy(hcol.data(), d->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(hval.data(), d->val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+      src_col = hcol.data();
+      src_val = hval.data();
+    }
+    std::vector<char> is_long(m, 0);
+    for (int r : long_rows) is_long[r] = 1;
+    int64_t ln = 0;
+    for (int r : long_rows) ln += rp[r + 1] - rp[r];
+    P->long_nnz = ln;
+    rcol.reserve(nnz - ln);
+    rval.reserve(nnz - ln);
+    lcol.reserve(ln);
+    lval.reserve(ln);
+    lrp.push_back(0);
+    std::vector<int32_t> rrp(m + 1, 0);
+    std::vector<std::pair<int32_t, double>> tmp;
+    for (int i = 0; i < m; i++) {
+      const int64_t b = rp[i], e = rp[i + 1];
+      if (is_long[i]) {
+        // columns sorted within the row (a stable sort keeps equal columns in order)
+        bool sorted = true;
+        for (int64_t j = b + 1; j < e && sorted; j++) sorted = src_col[j - 1] <= src_col[j];
+        if (sorted) {
+          lcol.insert(lcol.end(), src_col + b, src_col + e);
+          lval.insert(lval.end(), src_val + b, src_val + e);
+        } else {
+          tmp.clear();
+          for (int64_t j = b; j < e; j++) tmp.emplace_back(src_col[j], src_val[j]);
+          std::stable_sort(tmp.begin(), tmp.end(),
+                           [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &c) {
+                             return a.first < c.first;
+                           });
+          for (auto &t : tmp) {
+            lcol.push_back(t.first);
+            lval.push_back(t.second);
+          }
+        }
+        lrp.push_back((int32_t)lcol.size());
+      } else {
+        rcol.insert(rcol.end(), src_col + b, src_col + e);
+        rval.insert(rval.end(), src_val + b, src_val + e);
+      }
+      rrp[i + 1] = (int32_t)rcol.size();
+    }
+    rp.swap(rrp);
+    src_col = rcol.data();
+    src_val = rval.data();
+    src_dev = false;
+  }
+  const int64_t nnzr = rp[m];
+  P->nnz_reg = nnzr;
+  const hipMemcpyKind h2d = src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+
+  // ---- partitions of the regular part ----
   int kind = PART_NNZ, np = 1;
   if (d->mode == SPX_MODE_ROWS) {
     kind = PART_ROWS;
@@ -188,23 +303,20 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   } else if (d->mode == SPX_MODE_NNZ) {
     kind = PART_NNZ;
     np = std::max(1, d->parts);
-    if (nnz > 0 && np > nnz) np = (int)nnz;
-    if (nnz == 0) np = 1;
+    if (nnzr > 0 && np > nnzr) np = (int)nnzr;
+    if (nnzr == 0) np = 1;
   } else {
     kind = PART_TASKS;
-    if (d->nb <= 0) return fail(SPX_EARG, "nb must be positive");
-    int64_t T = nnz > 0 ? (nnz + d->nb - 1) / d->nb : 1;  // SB/src/dspmv_mgpu_v2.cu:218
+    int64_t T = nnzr > 0 ? (nnzr + d->nb - 1) / d->nb : 1;  // SB/src/dspmv_mgpu_v2.cu:218
     if (T > (1 << 24)) return fail(SPX_ERANGE, "too many tasks (nb too small)");
     np = (int)T;
   }
-  P->nparts = np;
 
   int rc;
-  if ((rc = P->alloc(&P->d_rowptr, m + 1))) return rc;
-  HIP_TRY(hipMemcpy(P->d_rowptr, rp.data(), sizeof(int32_t) * (m + 1), hipMemcpyHostToDevice));
+  if ((rc = P->upload(&P->d_rowptr, rp.data(), m + 1))) return rc;
   if ((rc = P->alloc(&P->d_parts, np))) return rc;
-  if (nnz == 0) {
-    // every row empty: one partition owning all rows
+  if (nnzr == 0) {
+    // every regular row empty: one partition owning all rows
     Part q = {};
     q.s = 0;
     q.e = -1;
@@ -212,13 +324,13 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     q.row_hi = m;
     P->parts.assign(1, q);
   } else {
-    HIP_TRY(launch_partition(P->d_rowptr, m, nnz, np, kind, P->d_parts, 0));
+    HIP_TRY(launch_partition(P->d_rowptr, m, nnzr, np, kind, P->d_parts, 0));
     P->parts.resize(np);
     HIP_TRY(hipMemcpy(P->parts.data(), P->d_parts, sizeof(Part) * np, hipMemcpyDeviceToHost));
   }
 
-  // per-kernel work decomposition
-  const double mean = m > 0 ? (double)nnz / m : 0;
+  // ---- per-kernel work decomposition ----
+  const double mean = m > 0 ? (double)nnzr / m : 0;
   if (P->kernel == SPX_KERNEL_VECTOR) {
     P->lanes = d->lanes > 0 ? d->lanes : default_lanes(mean);
     if (P->lanes > 64 || (P->lanes & (P->lanes - 1))) return fail(SPX_EARG, "lanes must be a power of two <= 64");
@@ -228,7 +340,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     if (std::find(std::begin(ok), std::end(ok), P->items) == std::end(ok))
       return fail(SPX_EARG, "items must be one of 2,4,6,8,12,16");
   } else {
-    P->sigma = d->sigma > 0 ? d->sigma : tile_sigma_auto(m, nnz);
+    P->sigma = d->sigma > 0 ? d->sigma : tile_sigma_auto(m, nnzr);
     if (P->sigma < 1 || P->sigma > 32) return fail(SPX_EARG, "sigma must be in 1..32");
   }
   const int tw = kBlock * P->items;
@@ -269,99 +381,105 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   P->ntiles = toff;
   P->ntp = tpoff;
   HIP_TRY(hipMemcpy(P->d_parts, P->parts.data(), sizeof(Part) * P->parts.size(), hipMemcpyHostToDevice));
-  P->any_carry = false;
-  for (auto &q : P->parts)
-    if (q.sflag || q.eflag) P->any_carry = true;
-  if (P->kernel != SPX_KERNEL_VECTOR && nnz > 0) P->any_carry = true;
 
-  // unit table
   std::vector<Unit> units(std::max<int64_t>(1, P->nunits));
   for (size_t pi = 0; pi < P->parts.size(); pi++)
     for (int64_t u = 0; u < P->parts[pi].units; u++) units[P->parts[pi].unit_off + u] = Unit{(int32_t)pi, (int32_t)u};
-  if ((rc = P->alloc(&P->d_units, P->nunits))) return rc;
-  if (P->nunits)
-    HIP_TRY(hipMemcpy(P->d_units, units.data(), sizeof(Unit) * P->nunits, hipMemcpyHostToDevice));
+  if ((rc = P->upload(&P->d_units, units.data(), P->nunits))) return rc;
 
   if ((rc = P->alloc(&P->d_crow, P->ncarry))) return rc;
   if ((rc = P->alloc(&P->d_cval, P->ncarry))) return rc;
   HIP_TRY(launch_fill_i32(P->d_crow, -1, P->ncarry, 0));
+  HIP_TRY(hipMemset(P->d_cval, 0, sizeof(double) * std::max<int64_t>(1, P->ncarry)));
 
-  // matrix payload
+  // ---- regular payload ----
   if (P->kernel == SPX_KERNEL_TILE) {
     const int64_t tot = P->ntiles * kOmega * P->sigma;
     int32_t *col_tmp = nullptr;
     double *val_tmp = nullptr;
-    if (nnz > 0) {
-      if (hipMalloc((void **)&col_tmp, sizeof(int32_t) * nnz) != hipSuccess ||
-          hipMalloc((void **)&val_tmp, sizeof(double) * nnz) != hipSuccess) {
+    if (nnzr > 0) {
+      if (hipMalloc((void **)&col_tmp, sizeof(int32_t) * nnzr) != hipSuccess ||
+          hipMalloc((void **)&val_tmp, sizeof(double) * nnzr) != hipSuccess) {
         if (col_tmp) (void)hipFree(col_tmp);
         return fail(SPX_EMEM, "hipMalloc: staging for tile conversion");
       }
-      HIP_TRY(hipMemcpy(col_tmp, d->col, sizeof(int32_t) * nnz, hipMemcpyDefault));
-      HIP_TRY(hipMemcpy(val_tmp, d->val, sizeof(double) * nnz, hipMemcpyDefault));
+      hipError_t e = hipMemcpy(col_tmp, src_col, sizeof(int32_t) * nnzr, h2d);
+      if (e == hipSuccess) e = hipMemcpy(val_tmp, src_val, sizeof(double) * nnzr, h2d);
+      if (e != hipSuccess) {
+        (void)hipFree(col_tmp);
+        (void)hipFree(val_tmp);
+        HIP_TRY(e);
+      }
     }
-    if ((rc = P->alloc(&P->d_tp, P->ntp)) || (rc = P->alloc(&P->d_tf, P->ntiles * kOmega)) ||
-        (rc = P->alloc(&P->d_col, tot))) {
-      (void)hipFree(col_tmp);
-      (void)hipFree(val_tmp);
-      return rc;
-    }
-    if (P->precision == SPX_PREC_F32)
-      rc = P->alloc(&P->d_val32, tot);
-    else
-      rc = P->alloc(&P->d_val, tot);
-    if (rc) {
-      (void)hipFree(col_tmp);
-      (void)hipFree(val_tmp);
-      return rc;
-    }
+    rc = P->alloc(&P->d_tp, P->ntp);
+    if (!rc) rc = P->alloc(&P->d_tf, P->ntiles * kOmega);
+    if (!rc) rc = P->alloc(&P->d_col, tot);
+    if (!rc) rc = P->precision == SPX_PREC_F32 ? P->alloc(&P->d_val32, tot) : P->alloc(&P->d_val, tot);
     hipError_t e = hipSuccess;
-    if (nnz > 0)
+    if (!rc && nnzr > 0)
       e = launch_tile_convert(P->d_rowptr, col_tmp, val_tmp, P->parts.data(), P->d_parts, (int)P->parts.size(),
                               P->sigma, P->d_tp, P->d_tf, P->d_col, P->d_val, P->d_val32, 0);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    (void)hipFree(col_tmp);
-    (void)hipFree(val_tmp);
+    if (!rc && e == hipSuccess) e = hipDeviceSynchronize();
+    if (col_tmp) (void)hipFree(col_tmp);
+    if (val_tmp) (void)hipFree(val_tmp);
+    if (rc) return rc;
     HIP_TRY(e);
-    // empty rows are written by their own small kernel
+    // empty rows (long rows included) are written by their own small kernel
     std::vector<int32_t> empty;
     for (int i = 0; i < m; i++)
       if (rp[i] == rp[i + 1]) empty.push_back(i);
     P->nempty = (int64_t)empty.size();
-    if ((rc = P->alloc(&P->d_empty, P->nempty))) return rc;
-    if (P->nempty)
-      HIP_TRY(hipMemcpy(P->d_empty, empty.data(), sizeof(int32_t) * P->nempty, hipMemcpyHostToDevice));
+    if ((rc = P->upload(&P->d_empty, empty.data(), P->nempty))) return rc;
   } else {
-    if ((rc = P->alloc(&P->d_col, nnz))) return rc;
-    if (nnz) HIP_TRY(hipMemcpy(P->d_col, d->col, sizeof(int32_t) * nnz, hipMemcpyDefault));
+    if ((rc = P->alloc(&P->d_col, nnzr))) return rc;
+    if (nnzr) HIP_TRY(hipMemcpy(P->d_col, src_col, sizeof(int32_t) * nnzr, h2d));
     if (P->precision == SPX_PREC_F32) {
       double *tmp = nullptr;
-      if ((rc = P->alloc(&P->d_val32, nnz))) return rc;
-      if (nnz) {
-        HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * nnz));
-        hipError_t e = hipMemcpy(tmp, d->val, sizeof(double) * nnz, hipMemcpyDefault);
-        if (e == hipSuccess) e = launch_to_f32(tmp, P->d_val32, nnz, 0);
+      if ((rc = P->alloc(&P->d_val32, nnzr))) return rc;
+      if (nnzr) {
+        HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * nnzr));
+        hipError_t e = hipMemcpy(tmp, src_val, sizeof(double) * nnzr, h2d);
+        if (e == hipSuccess) e = launch_to_f32(tmp, P->d_val32, nnzr, 0);
         if (e == hipSuccess) e = hipDeviceSynchronize();
         (void)hipFree(tmp);
         HIP_TRY(e);
       }
     } else {
-      if ((rc = P->alloc(&P->d_val, nnz))) return rc;
-      if (nnz) HIP_TRY(hipMemcpy(P->d_val, d->val, sizeof(double) * nnz, hipMemcpyDefault));
+      if ((rc = P->alloc(&P->d_val, nnzr))) return rc;
+      if (nnzr) HIP_TRY(hipMemcpy(P->d_val, src_val, sizeof(double) * nnzr, h2d));
     }
     if (P->kernel == SPX_KERNEL_MERGE) {
       if ((rc = P->alloc(&P->d_coords, P->ncoords))) return rc;
-      if (nnz > 0 || m > 0)
-        HIP_TRY(launch_merge_coords(P->d_rowptr, P->d_parts, (int)P->parts.size(), P->ncoords, tw, P->d_coords, 0));
+      HIP_TRY(launch_merge_coords(P->d_rowptr, P->d_parts, (int)P->parts.size(), P->ncoords, tw, P->d_coords, 0));
     }
   }
   if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
 
-  // launch layout: partitions -> streams
+  // ---- long-row payload ----
+  if (P->nlong > 0) {
+    P->nblocks = (int)(((int64_t)n + kColBlock - 1) / kColBlock);
+    if ((rc = P->upload(&P->d_lrowptr, lrp.data(), P->nlong + 1))) return rc;
+    if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
+    if ((rc = P->upload(&P->d_lcol, lcol.data(), P->long_nnz))) return rc;
+    if (P->precision == SPX_PREC_F32) {
+      std::vector<float> l32(lval.begin(), lval.end());
+      if ((rc = P->upload(&P->d_lval32, l32.data(), P->long_nnz))) return rc;
+    } else {
+      if ((rc = P->upload(&P->d_lval, lval.data(), P->long_nnz))) return rc;
+    }
+    if ((rc = P->alloc(&P->d_seg, (int64_t)(P->nblocks + 1) * P->nlong))) return rc;
+    if ((rc = P->alloc(&P->d_partial, (int64_t)P->nblocks * P->nlong))) return rc;
+    HIP_TRY(launch_long_seg(P->d_lrowptr, P->d_lcol, P->nlong, P->nblocks, P->d_seg, 0));
+    HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&P->ldone, hipEventDisableTiming));
+  }
+
+  // ---- launch layout: partitions -> streams ----
   int nstreams = 1;
   if (P->mode == SPX_MODE_QUEUE) {
     nstreams = std::max(1, std::min(P->queues, (int)P->parts.size()));
-    if ((rc = P->alloc(&P->d_queue, nstreams))) return rc;
+    if ((rc = P->alloc(&P->d_queue, (int64_t)nstreams * kXcd))) return rc;
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -372,7 +490,10 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
       if (t1 <= t0) continue;
       int64_t ub = P->parts[t0].unit_off;
       int64_t ue = P->parts[t1 - 1].unit_off + P->parts[t1 - 1].units;
-      int grid = (int)std::min<int64_t>(std::max<int64_t>(1, ue - ub), (int64_t)cus * 8 / nstreams + 1);
+      if (ue <= ub) continue;
+      // persistent workgroups: a multiple of the XCD count, ~8 per CU overall
+      int64_t want = std::max<int64_t>(kXcd, (int64_t)cus * 8 / nstreams);
+      int grid = (int)std::min<int64_t>(direct_grid(ue - ub), direct_grid(want));
       P->launches.push_back({c, ub, ue - ub, grid});
     }
   } else {
@@ -381,7 +502,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     for (size_t pi = 0; pi < P->parts.size(); pi++) {
       const Part &q = P->parts[pi];
       if (q.units == 0) continue;
-      P->launches.push_back({(int)(pi % nstreams), q.unit_off, q.units, (int)std::min<int64_t>(q.units, 1 << 30)});
+      P->launches.push_back({(int)(pi % nstreams), q.unit_off, q.units, direct_grid(q.units)});
     }
   }
   if (nstreams > 1) {
@@ -393,9 +514,39 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     }
     HIP_TRY(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
   }
+
+  // ---- carry runs from a dry execute ----
+  if (P->ncarry > 0 && nnzr > 0) {
+    double *x0 = nullptr, *y0 = nullptr;
+    HIP_TRY(hipMalloc((void **)&x0, sizeof(double) * std::max(n, 1)));
+    hipError_t e = hipMalloc((void **)&y0, sizeof(double) * m);
+    if (e == hipSuccess) e = hipMemset(x0, 0, sizeof(double) * std::max(n, 1));
+    int erc = e == hipSuccess ? execute_impl(P, 1.0, x0, 0.0, y0, 0, nullptr, nullptr, true) : SPX_EDEV;
+    if (erc == SPX_OK) e = hipDeviceSynchronize();
+    std::vector<int32_t> crow(P->ncarry);
+    if (erc == SPX_OK && e == hipSuccess)
+      e = hipMemcpy(crow.data(), P->d_crow, sizeof(int32_t) * P->ncarry, hipMemcpyDeviceToHost);
+    (void)hipFree(x0);
+    if (y0) (void)hipFree(y0);
+    if (erc != SPX_OK) return erc;
+    HIP_TRY(e);
+    std::vector<Run> runs;
+    for (int64_t k = 0; k < P->ncarry; k++) {
+      const int32_t r = crow[k];
+      if (r < 0) continue;
+      if (!runs.empty() && runs.back().row == r)
+        runs.back().end = (int32_t)(k + 1);
+      else
+        runs.push_back(Run{r, (int32_t)k, (int32_t)(k + 1)});
+    }
+    P->nruns = (int64_t)runs.size();
+    if ((rc = P->upload(&P->d_runs, runs.data(), P->nruns))) return rc;
+  }
   HIP_TRY(hipDeviceSynchronize());
   return SPX_OK;
 }
+
+extern "C" {
 
 int spx_plan_create(spx_plan_t *out, const spx_plan_desc *d) {
   if (!out || !d) return fail(SPX_EARG, "null argument");
@@ -420,86 +571,115 @@ static hipError_t launch_main(spx_plan *P, const KArgs<V, X> &base, const spx_pl
   a.unit_begin = L.ubeg;
   a.unit_count = L.ucnt;
   a.queue = nullptr;
-  int grid = (int)std::min<int64_t>(L.ucnt, 1 << 30);
   if (P->mode == SPX_MODE_QUEUE) {
-    a.queue = P->d_queue + L.stream;
-    hipError_t e = launch_reset(a.queue, 1, st);
+    a.queue = P->d_queue + (int64_t)L.stream * kXcd;
+    hipError_t e = launch_reset(a.queue, kXcd, st);
     if (e != hipSuccess) return e;
-    grid = L.grid;
   }
   if constexpr (std::is_same<V, double>::value) {
-    if (P->kernel == SPX_KERNEL_VECTOR) return launch_vector_f64(a, P->lanes, grid, st);
-    if (P->kernel == SPX_KERNEL_MERGE) return launch_merge_f64(a, P->items, grid, st);
-    return launch_tile_f64(a, grid, st);
+    if (P->kernel == SPX_KERNEL_VECTOR) return launch_vector_f64(a, P->lanes, L.grid, st);
+    if (P->kernel == SPX_KERNEL_MERGE) return launch_merge_f64(a, P->items, L.grid, st);
+    return launch_tile_f64(a, L.grid, st);
   } else {
-    if (P->kernel == SPX_KERNEL_VECTOR) return launch_vector_f32(a, P->lanes, grid, st);
-    if (P->kernel == SPX_KERNEL_MERGE) return launch_merge_f32(a, P->items, grid, st);
-    return launch_tile_f32(a, grid, st);
+    if (P->kernel == SPX_KERNEL_VECTOR) return launch_vector_f32(a, P->lanes, L.grid, st);
+    if (P->kernel == SPX_KERNEL_MERGE) return launch_merge_f32(a, P->items, L.grid, st);
+    return launch_tile_f32(a, L.grid, st);
   }
 }
 
 template <typename V, typename X>
-static hipError_t run_all(spx_plan *P, const KArgs<V, X> &a, hipStream_t st, hipEvent_t t0, hipEvent_t t1) {
+static hipError_t run_regular(spx_plan *P, const KArgs<V, X> &a, hipStream_t st) {
   hipError_t e = hipSuccess;
-  if (t0) e = hipEventRecord(t0, st);
-  if (e != hipSuccess) return e;
   if (P->streams.empty()) {
     for (auto &L : P->launches)
       if ((e = launch_main(P, a, L, st)) != hipSuccess) return e;
-  } else {
-    if ((e = hipEventRecord(P->fork, st)) != hipSuccess) return e;
-    std::vector<char> used(P->streams.size(), 0);
-    for (auto &L : P->launches) {
-      hipStream_t s = P->streams[L.stream];
-      if (!used[L.stream]) {
-        if ((e = hipStreamWaitEvent(s, P->fork, 0)) != hipSuccess) return e;
-        used[L.stream] = 1;
-      }
-      if ((e = launch_main(P, a, L, s)) != hipSuccess) return e;
-    }
-    for (size_t c = 0; c < P->streams.size(); c++) {
-      if (!used[c]) continue;
-      if ((e = hipEventRecord(P->done[c], P->streams[c])) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(st, P->done[c], 0)) != hipSuccess) return e;
-    }
+    return e;
   }
-  if (t1) e = hipEventRecord(t1, st);
+  if ((e = hipEventRecord(P->fork, st)) != hipSuccess) return e;
+  std::vector<char> used(P->streams.size(), 0);
+  for (auto &L : P->launches) {
+    hipStream_t s = P->streams[L.stream];
+    if (!used[L.stream]) {
+      if ((e = hipStreamWaitEvent(s, P->fork, 0)) != hipSuccess) return e;
+      used[L.stream] = 1;
+    }
+    if ((e = launch_main(P, a, L, s)) != hipSuccess) return e;
+  }
+  for (size_t c = 0; c < P->streams.size(); c++) {
+    if (!used[c]) continue;
+    if ((e = hipEventRecord(P->done[c], P->streams[c])) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(st, P->done[c], 0)) != hipSuccess) return e;
+  }
   return e;
 }
 
+template <typename V, typename X>
+static KArgs<V, X> make_args(spx_plan *P, const V *val, const X *x, double alpha, double beta, double *y) {
+  KArgs<V, X> a = {};
+  a.parts = P->d_parts;
+  a.units = P->d_units;
+  a.rowptr = P->d_rowptr;
+  a.col = P->d_col;
+  a.val = val;
+  a.x = x;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.y = y;
+  a.crow = P->d_crow;
+  a.cval = P->d_cval;
+  a.coords = P->d_coords;
+  a.tile_ptr = P->d_tp;
+  a.tile_flags = P->d_tf;
+  a.sigma = P->sigma;
+  return a;
+}
+
+// Enqueue one SpMV.  t0/t1 (optional) bracket the SpMV kernels proper (the
+// regular partitions and the long-row sweep, which overlap); x splitting,
+// the fix-up and the long-row reduction fall outside.  `dry` runs only the
+// regular kernels (used once at plan time to record the carry runs).
 static int execute_impl(spx_plan *P, double alpha, const double *x, double beta, double *y, hipStream_t st,
-                        hipEvent_t t0, hipEvent_t t1) {
+                        hipEvent_t t0, hipEvent_t t1, bool dry) {
   if (!P) return fail(SPX_EARG, "null plan");
   if (P->m == 0) return SPX_OK;
   if (!y || (P->n > 0 && !x)) return fail(SPX_EARG, "null x or y");
-  if (P->kernel == SPX_KERNEL_TILE) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
-  if (P->nnz == 0 && P->kernel == SPX_KERNEL_TILE) return SPX_OK;
-  hipError_t e;
-  if (P->precision == SPX_PREC_F32) {
-    HIP_TRY(launch_split_x(x, P->d_xhl, P->n, st));
-    KArgs<float, XPair> a = {};
-    a.parts = P->d_parts; a.units = P->d_units; a.rowptr = P->d_rowptr; a.col = P->d_col;
-    a.val = P->d_val32; a.x = P->d_xhl; a.alpha = alpha; a.beta = beta; a.y = y;
-    a.crow = P->d_crow; a.cval = P->d_cval; a.coords = P->d_coords; a.tile_ptr = P->d_tp;
-    a.tile_flags = P->d_tf; a.sigma = P->sigma;
-    e = run_all(P, a, st, t0, t1);
-  } else {
-    KArgs<double, double> a = {};
-    a.parts = P->d_parts; a.units = P->d_units; a.rowptr = P->d_rowptr; a.col = P->d_col;
-    a.val = P->d_val; a.x = x; a.alpha = alpha; a.beta = beta; a.y = y;
-    a.crow = P->d_crow; a.cval = P->d_cval; a.coords = P->d_coords; a.tile_ptr = P->d_tp;
-    a.tile_flags = P->d_tf; a.sigma = P->sigma;
-    e = run_all(P, a, st, t0, t1);
+  const bool f32 = P->precision == SPX_PREC_F32;
+  if (f32) HIP_TRY(launch_split_x(x, P->d_xhl, P->n, st));
+  if (t0) HIP_TRY(hipEventRecord(t0, st));
+  const bool longpath = P->nlong > 0 && !dry;
+  if (longpath) {
+    HIP_TRY(hipEventRecord(P->lfork, st));
+    HIP_TRY(hipStreamWaitEvent(P->lstream, P->lfork, 0));
+    if (f32)
+      HIP_TRY(launch_long_f32(P->d_seg, P->nlong, P->nblocks, P->d_lcol, P->d_lval32, P->d_xhl, P->n, P->d_partial,
+                              P->lstream));
+    else
+      HIP_TRY(launch_long_f64(P->d_seg, P->nlong, P->nblocks, P->d_lcol, P->d_lval, x, P->n, P->d_partial,
+                              P->lstream));
+    HIP_TRY(hipEventRecord(P->ldone, P->lstream));
   }
-  HIP_TRY(e);
-  if (P->any_carry) HIP_TRY(launch_fixup(P->d_crow, P->d_cval, P->ncarry, alpha, beta, y, st));
+  if (P->kernel == SPX_KERNEL_TILE) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
+  const bool regular = !(P->nnz_reg == 0 && P->kernel == SPX_KERNEL_TILE);
+  if (regular) {
+    hipError_t e;
+    if (f32)
+      e = run_regular(P, make_args<float, XPair>(P, P->d_val32, P->d_xhl, alpha, beta, y), st);
+    else
+      e = run_regular(P, make_args<double, double>(P, P->d_val, x, alpha, beta, y), st);
+    HIP_TRY(e);
+  }
+  if (longpath) HIP_TRY(hipStreamWaitEvent(st, P->ldone, 0));
+  if (t1) HIP_TRY(hipEventRecord(t1, st));
+  if (dry) return SPX_OK;
+  HIP_TRY(launch_fixup(P->d_runs, P->nruns, P->d_crow, P->d_cval, alpha, beta, y, st));
+  if (longpath) HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->nblocks, P->d_lrows, alpha, y, st));
   return SPX_OK;
 }
 
 extern "C" {
 
 int spx_plan_execute(spx_plan_t p, double alpha, const double *x, double beta, double *y, void *stream) {
-  return execute_impl(p, alpha, x, beta, y, (hipStream_t)stream, nullptr, nullptr);
+  return execute_impl(p, alpha, x, beta, y, (hipStream_t)stream, nullptr, nullptr, false);
 }
 
 int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, int64_t ldx, double beta, double *Y,
@@ -507,7 +687,7 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
   if (!p || k < 0 || ldx < p->n || ldy < p->m) return fail(SPX_EARG, "bad multi-RHS arguments");
   for (int j = 0; j < k; j++) {
     int rc = execute_impl(p, alpha, X + (size_t)j * ldx, beta, Y + (size_t)j * ldy, (hipStream_t)stream, nullptr,
-                          nullptr);
+                          nullptr, false);
     if (rc) return rc;
   }
   return SPX_OK;
@@ -515,8 +695,10 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
 
 int spx_plan_info(spx_plan_t p, int64_t *out, int cap) {
   if (!p || !out) return 0;
-  int64_t v[] = {p->m, p->n, p->nnz, p->kernel, p->mode, (int64_t)p->parts.size(), p->sigma, p->lanes,
-                 p->items, p->nunits, p->ncarry, p->ntiles, p->bytes, p->precision, p->queues};
+  int64_t v[] = {p->m,      p->n,        p->nnz,       p->kernel, p->mode,   (int64_t)p->parts.size(),
+                 p->sigma,  p->lanes,    p->items,     p->nunits, p->ncarry, p->ntiles,
+                 p->bytes,  p->precision, p->queues,   p->nlong,  p->long_nnz, p->nblocks,
+                 p->nruns,  p->long_th};
   int c = std::min<int>(cap, (int)(sizeof(v) / sizeof(v[0])));
   for (int i = 0; i < c; i++) out[i] = v[i];
   return c;
@@ -540,23 +722,23 @@ int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap) {
     case 2: src = p->d_tf; bytes = p->ntiles * kOmega * 4; break;
     case 3: src = p->kernel == SPX_KERNEL_TILE ? p->d_col : nullptr; bytes = p->ntiles * kOmega * p->sigma * 4; break;
     case 4: src = p->kernel == SPX_KERNEL_TILE ? p->d_val : nullptr; bytes = p->ntiles * kOmega * p->sigma * 8; break;
-    case 5: src = p->d_coords; bytes = p->ncoords * 8; break;
+    case 5: {
+      if (!p->d_coords) return 0;
+      std::vector<Coord> c((size_t)p->ncoords);
+      if (dst && p->ncoords &&
+          hipMemcpy(c.data(), p->d_coords, sizeof(Coord) * p->ncoords, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+      for (auto &x : c) { tmp.push_back(x.row); tmp.push_back(x.nz); }
+      bytes = p->ncoords * 16;
+      if (dst) std::memcpy(dst, tmp.data(), (size_t)std::min(bytes, cap));
+      return bytes;
+    }
     default: return -1;
   }
   if (!src) return 0;
-  if (dst && bytes > 0) {
-    if (what == 5) {
-      std::vector<Coord> c((size_t)p->ncoords);
-      if (hipMemcpy(c.data(), src, (size_t)bytes, hipMemcpyDeviceToHost) != hipSuccess) return -1;
-      std::vector<int64_t> o;
-      for (auto &x : c) { o.push_back(x.row); o.push_back(x.nz); }
-      int64_t ob = (int64_t)o.size() * 8;
-      std::memcpy(dst, o.data(), (size_t)std::min(ob, cap));
-      return ob;
-    }
-    if (hipMemcpy(dst, src, (size_t)std::min(bytes, cap), hipMemcpyDeviceToHost) != hipSuccess) return -1;
-  }
-  return what == 5 ? bytes * 2 : bytes;
+  if (dst && bytes > 0 && hipMemcpy(dst, src, (size_t)std::min(bytes, cap), hipMemcpyDeviceToHost) != hipSuccess)
+    return -1;
+  return bytes;
 }
 
 int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, double *y, int warmup, int reps,
@@ -573,11 +755,11 @@ int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, doub
   void *flush = nullptr;
   if (rc == SPX_OK && flush_bytes > 0 && hipMalloc(&flush, (size_t)flush_bytes) != hipSuccess)
     rc = fail(SPX_EMEM, "hipMalloc of the cache-flush buffer failed");
-  for (int i = 0; i < warmup && rc == SPX_OK; i++) rc = execute_impl(p, alpha, x, beta, y, st, nullptr, nullptr);
+  for (int i = 0; i < warmup && rc == SPX_OK; i++) rc = execute_impl(p, alpha, x, beta, y, st, nullptr, nullptr, false);
   for (int i = 0; i < reps && rc == SPX_OK; i++) {
     if (flush) (void)hipMemsetAsync(flush, i & 0xff, (size_t)flush_bytes, st);
     (void)hipEventRecord(ev[4 * i], st);
-    rc = execute_impl(p, alpha, x, beta, y, st, ev[4 * i + 1], ev[4 * i + 2]);
+    rc = execute_impl(p, alpha, x, beta, y, st, ev[4 * i + 1], ev[4 * i + 2], false);
     (void)hipEventRecord(ev[4 * i + 3], st);
   }
   if (hipStreamSynchronize(st) != hipSuccess && rc == SPX_OK) rc = fail(SPX_EDEV, "stream synchronize failed");

@@ -12,6 +12,14 @@ namespace spx {
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64s)
 constexpr int kWave = 64;     // CDNA wavefront width
 constexpr int kOmega = 64;    // lanes per CSR5-style tile
+constexpr int kXcd = 8;       // accelerator complex dies (one L2 each)
+constexpr int kColBlock = 4096;  // long-row path: x entries staged in LDS per block
+
+// Workgroups for a direct launch over `count` units: a multiple of kXcd so
+// every XCD receives the same contiguous share.
+__host__ __device__ inline int direct_grid(int64_t count) {
+  return (int)(((count + kXcd - 1) / kXcd) * kXcd);
+}
 
 // One nnz-contiguous partition of the matrix (a stream's share in the v1
 // mode, a task in the v2 mode, a row block in the baseline mode).
@@ -42,6 +50,11 @@ struct Unit {
 // Merge-path coordinate of a unit boundary: absolute row, absolute nonzero.
 struct Coord {
   int32_t row, nz;
+};
+
+// A run of carry slots that belong to one split row.
+struct Run {
+  int32_t row, beg, end;
 };
 
 struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
@@ -89,8 +102,19 @@ hipError_t launch_split_x(const double *x, XPair *xhl, int64_t n,
 hipError_t launch_to_f32(const double *v, float *o, int64_t n, hipStream_t st);
 hipError_t launch_empty_rows(const int32_t *rows, int64_t nrows, double beta,
                              double *y, hipStream_t st);
-hipError_t launch_fixup(const int32_t *crow, const double *cval, int64_t nc,
-                        double alpha, double beta, double *y, hipStream_t st);
+hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow,
+                        const double *cval, double alpha, double beta, double *y,
+                        hipStream_t st);
+hipError_t launch_long_seg(const int32_t *lrowptr, const int32_t *lcol, int nlong,
+                           int nblocks, int32_t *seg, hipStream_t st);
+hipError_t launch_long_f64(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol,
+                           const double *lval, const double *x, int n, double *partial,
+                           hipStream_t st);
+hipError_t launch_long_f32(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol,
+                           const float *lval, const XPair *x, int n, double *partial,
+                           hipStream_t st);
+hipError_t launch_long_reduce(const double *partial, int nlong, int nblocks,
+                              const int32_t *rows, double alpha, double *y, hipStream_t st);
 hipError_t launch_reset(int *q, int n, hipStream_t st);
 hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag,
                              hipStream_t st);

@@ -36,7 +36,7 @@ class PlanDesc(C.Structure):
         ("kernel", C.c_int), ("mode", C.c_int), ("parts", C.c_int),
         ("nb", C.c_int64), ("queues", C.c_int), ("sigma", C.c_int),
         ("lanes", C.c_int), ("items", C.c_int), ("precision", C.c_int),
-        ("flags", C.c_int),
+        ("flags", C.c_int), ("long_row", C.c_int64),
     ]
 
 
@@ -165,7 +165,7 @@ class Plan:
     """Device-resident SpMV plan (spx_plan_create / execute / destroy)."""
 
     def __init__(self, rowptr, col, val, n=None, kernel=KERNEL_MERGE, mode=MODE_NNZ, parts=1, nb=0,
-                 queues=1, sigma=0, lanes=0, items=0, precision=PREC_F64):
+                 queues=1, sigma=0, lanes=0, items=0, precision=PREC_F64, long_row=0):
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
         self.col = np.ascontiguousarray(col, dtype=np.int32)
         self.val = np.ascontiguousarray(val, dtype=np.float64)
@@ -180,6 +180,7 @@ class Plan:
         d.val = self.val.ctypes.data if len(self.val) else None
         d.kernel, d.mode, d.parts, d.nb, d.queues = kernel, mode, parts, nb, queues
         d.sigma, d.lanes, d.items, d.precision = sigma, lanes, items, precision
+        d.long_row = long_row
         h = C.c_void_p()
         check(lib().spx_plan_create(C.byref(h), C.byref(d)))
         self.h = h
@@ -204,10 +205,11 @@ class Plan:
         return yd.download(np.float64, self.m)
 
     def info(self):
-        out = (C.c_int64 * 16)()
-        k = lib().spx_plan_info(self.h, out, 16)
+        out = (C.c_int64 * 32)()
+        k = lib().spx_plan_info(self.h, out, 32)
         keys = ["m", "n", "nnz", "kernel", "mode", "parts", "sigma", "lanes", "items", "units",
-                "carries", "tiles", "bytes", "precision", "queues"]
+                "carries", "tiles", "bytes", "precision", "queues", "nlong", "long_nnz", "col_blocks",
+                "runs", "long_row"]
         return {keys[i]: int(out[i]) for i in range(k)}
 
     def export(self, what):

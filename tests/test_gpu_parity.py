@@ -307,3 +307,50 @@ def test_precision_form(name, kernel):
     if name == "rail":
         # values are exact in FP32; x as a (hi, lo) pair keeps ~48 bits
         assert rel <= 1e-13
+
+
+# ---- long rows (column-blocked path with x staged in LDS) ------------------------------
+
+@pytest.mark.parametrize("long_row", [16, 300, -1])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("mc", [dict(parts=1), dict(parts=4), dict(mode=spx.MODE_QUEUE, nb=3000, queues=2)],
+                         ids=["p1", "p4", "queue"])
+def test_long_row_path(long_row, kernel, mc):
+    A = W.zipf(9000, 1.1, 3000, seed=21)
+    x = W.vector(A["n"], 5)
+    y0 = W.vector(A["m"], 6)
+    y, P = run_plan(A, x, 0.8, 0.6, y0, kernel=kernel, long_row=long_row, **mc)
+    info = P.info()
+    lens = np.diff(A["rowptr"])
+    if long_row > 0:
+        assert info["nlong"] == int((lens >= long_row).sum()) > 0
+    else:
+        assert info["nlong"] == 0
+    check_fp64(A, x, y, 0.8, 0.6, y0)
+
+
+def test_long_rows_unsorted_columns_and_bit_exact_integers():
+    A = W.small_int(2000, 9000, 40, seed=3)
+    rng = np.random.default_rng(1)
+    rp = A["rowptr"]
+    col = A["col"].copy()
+    val = A["val"].copy()
+    for r in range(0, A["m"], 7):  # shuffle some rows: the long-row split must sort them
+        idx = rng.permutation(np.arange(rp[r], rp[r + 1]))
+        col[rp[r]:rp[r + 1]] = col[idx]
+        val[rp[r]:rp[r + 1]] = val[idx]
+    B = W.finish(rp, col, val, A["m"], A["n"])
+    x = W.vector(B["n"], 9, W.VAL_SMALLINT)
+    for kernel in (1, 2, 3):
+        y, P = run_plan(B, x, 1.0, 0.0, None, kernel=kernel, long_row=50)
+        assert P.info()["nlong"] > 0
+        assert np.array_equal(y, oracle.spmv(rp, col, val, x, 1.0, 0.0))
+
+
+def test_long_row_path_precision_form():
+    A = W.zipf(6000, 1.1, 3000, seed=22)
+    x = W.vector(A["n"], 7)
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    y, P = run_plan(A, x, kernel=2, long_row=100, precision=spx.PREC_F32)
+    assert P.info()["nlong"] > 0
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-7
