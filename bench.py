@@ -42,7 +42,7 @@ def main():
     ap.add_argument("--parts", type=int, default=1)
     ap.add_argument("--items", type=int, default=0)
     ap.add_argument("--sigma", type=int, default=0)
-    ap.add_argument("--window", type=int, default=32768,
+    ap.add_argument("--window", type=int, default=1024,
                     help="half-width of the regular rows' column band (0 = uniform columns)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)

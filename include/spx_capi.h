@@ -131,6 +131,37 @@ int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta,
                   double *y, int warmup, int reps, int64_t flush_bytes,
                   double *out_ms);
 
+/* ---- batched form -----------------------------------------------------------
+ * Many independent SpMVs y_i = alpha*A_i*x_i + beta*y_i as ONE plan: the
+ * matrices are packed block-diagonally (columns shifted by the x offsets),
+ * so one launch (persistent workgroups in SPX_MODE_QUEUE) covers them all.
+ * x_off/y_off (count+1 entries each, may be NULL) receive where x_i and y_i
+ * start inside the concatenated x (sum n_i) and y (sum m_i). */
+typedef struct spx_batch_item {
+  int m, n;
+  int64_t nnz;
+  const int64_t *rowptr;    /* host, m+1 */
+  const int32_t *col;       /* host, nnz, 0-based within A_i */
+  const double *val;        /* host, nnz */
+} spx_batch_item;
+
+int spx_batch_create(spx_plan_t *out, int count, const spx_batch_item *items,
+                     const spx_plan_desc *tmpl, int64_t *x_off, int64_t *y_off);
+
+/* ---- HIP graphs ---------------------------------------------------------------
+ * Capture one execute of a plan (fork/join streams, kernels, fix-up) into an
+ * instantiated hipGraph with alpha, beta, x and y baked in; replay it with
+ * spx_graph_launch.  Nothing in the graph is a memset node (counters are
+ * cleared by a kernel). */
+typedef struct spx_graph *spx_graph_t;
+int spx_graph_capture(spx_plan_t p, double alpha, const double *x, double beta,
+                      double *y, spx_graph_t *out);
+int spx_graph_launch(spx_graph_t g, void *stream);
+int spx_graph_destroy(spx_graph_t g);
+/* Average ms per replay of `reps` back-to-back launches on a private stream
+ * (events around the whole run), after `warmup` replays. */
+int spx_graph_time(spx_graph_t g, int warmup, int reps, double *out_ms);
+
 /* ---- reference drop-in entry points (host pointers) ----------------------
  * Arguments and return codes as SparseBLAS/include/spmv_kernel.h:4-34. */
 int spx_dropin_baseline(int m, int n, long long nnz, double *alpha,

@@ -11,7 +11,7 @@
 // reproducible; no atomics touch y.
 //
 // beta == 0 never reads y (BLAS convention), so y may hold NaN on input.
-#include "spx_internal.h"
+#include "spx_device.h"
 
 namespace spx {
 
@@ -31,19 +31,6 @@ __device__ inline int row_search(const int32_t *rowptr, int lo, int hi, int64_t 
 
 __device__ inline double y_update(double alpha, double beta, double sum, const double *y, int r) {
   return beta == 0.0 ? alpha * sum : alpha * sum + beta * y[r];
-}
-
-// FP64 product.
-__device__ inline double prod(double v, const double *x, int c) { return v * x[c]; }
-
-// FP32 value times an FP32 (hi, lo) pair, returned as the FP64 sum of the
-// exact product v*hi (split by FMA into ph + eh) and the rounded v*lo.
-__device__ inline double prod(float v, const XPair *x, int c) {
-  XPair p = x[c];
-  float ph = v * p.hi;
-  float eh = __builtin_fmaf(v, p.hi, -ph);
-  float pl = __builtin_fmaf(v, p.lo, eh);
-  return (double)ph + (double)pl;
 }
 
 // Index of the last partition whose `off` member is <= g (parts sorted by off).
@@ -85,44 +72,6 @@ __device__ inline int64_t local_row_search(const int32_t *rowptr, const Part &q,
 __device__ inline void put_carry(int32_t *crow, double *cval, int64_t slot, int row, double v) {
   crow[slot] = row;
   cval[slot] = v;
-}
-
-// The dispatcher hands workgroup b to XCD b % kXcd.  Units are numbered in
-// nonzero order, so giving each XCD one contiguous range of units keeps the
-// x entries a unit gathers in that XCD's own L2 (neighbouring units touch
-// neighbouring columns for banded rows).
-__device__ inline int64_t xcd_unit(int64_t b, int64_t count) {
-  const int64_t per = (count + kXcd - 1) / kXcd;
-  return (b % kXcd) * per + b / kXcd;
-}
-
-// Next unit for this workgroup, uniform across the workgroup.  Direct
-// launches (grid = kXcd * ceil(count / kXcd)) take one unit each.  Persistent
-// workgroups pop units from a device work queue: one counter per XCD over
-// that XCD's contiguous share, and when the own share is drained they steal
-// from the other XCDs' shares in turn.  Returns >= count when done.
-__device__ inline int64_t next_unit(int *queue, int64_t iter, int64_t count, int *s_slot) {
-  if (!queue) return iter == 0 ? xcd_unit(blockIdx.x, count) : count;
-  if (threadIdx.x == 0) {
-    const int64_t per = (count + kXcd - 1) / kXcd;
-    const int home = blockIdx.x % kXcd;
-    int64_t u = count;
-    for (int k = 0; k < kXcd; k++) {
-      const int x = (home + k) % kXcd;
-      const int64_t lim = per * (x + 1) < count ? per * (x + 1) : count;
-      if ((int64_t)per * x >= lim) continue;
-      const int64_t got = per * x + atomicAdd(&queue[x], 1);
-      if (got < lim) {
-        u = got;
-        break;
-      }
-    }
-    *s_slot = (int)(u < count ? u : count);
-  }
-  __syncthreads();
-  const int64_t u = *s_slot;
-  __syncthreads();
-  return u;
 }
 
 // ---- partitioner ---------------------------------------------------------------
@@ -343,60 +292,170 @@ __global__ void k_fixup(const Run *__restrict__ runs, int64_t nruns, const int32
   y[R.row] = y_update(alpha, beta, s, y, R.row);
 }
 
-// ---- long rows: column blocks with x staged in LDS -----------------------------------
-// Rows with very many nonzeros at scattered columns make every x gather a
-// cache miss.  Those rows are kept apart (columns sorted per row) and swept
-// block by block over the columns: a workgroup stages x[c0, c0 + kColBlock)
-// in LDS with coalesced loads, then each wave takes a long row's slice of
-// nonzeros inside that column block (found by binary search at plan time)
-// and reduces it.  Per-(block, row) partials are summed per row in block
-// order by k_long_reduce, so the result is deterministic.
-
-__global__ void k_long_seg(const int32_t *__restrict__ lrowptr, const int32_t *__restrict__ lcol, int nlong,
-                           int nblocks, int32_t *__restrict__ seg) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)(nblocks + 1) * nlong) return;
-  const int b = (int)(g / nlong), r = (int)(g % nlong);
-  int lo = lrowptr[r], hi = lrowptr[r + 1];
-  const int64_t key = (int64_t)b * kColBlock;
-  while (lo < hi) {
-    int mid = lo + ((hi - lo) >> 1);
-    if ((int64_t)lcol[mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  seg[g] = lo;
-}
+// ---- long rows: column blocks with x staged in LDS --------------------------------
+// Rows with very many nonzeros at scattered columns make every x gather an
+// L2 request.  They are kept apart in a block-major CSR: slice (b, r) holds
+// the nonzeros of long row r whose columns fall in column block b, slices
+// ordered by block first (arrays padded by kLongPad entries).  One
+// workgroup per column block stages x[b*kColBlock, (b+1)*kColBlock) and the
+// block's slice pointers in LDS, then streams the block's nonzeros in
+// chunks of kBlock*8: each thread owns 8 consecutive, 8-aligned nonzeros
+// (16-byte vector loads), gathers x from LDS, walks the slice boundaries,
+// and a wave-shuffle segmented scan hands each slice piece to the thread that
+// closes it.  Pieces are added into an LDS accumulator per slice in a fixed
+// order, so the result is deterministic.  partial[b * nlong + r] is folded
+// per row by k_long_reduce.
 
 __device__ inline double prod_lds(double v, const double *sx, int c) { return v * sx[c]; }
 __device__ inline double prod_lds(float v, const XPair *sx, int c) { return prod(v, sx, c); }
 
-template <typename V, typename X>
-__global__ __launch_bounds__(kBlock) void k_long(const int32_t *__restrict__ seg, int nlong, int nblocks,
-                                                 const int32_t *__restrict__ lcol, const V *__restrict__ lval,
-                                                 const X *__restrict__ x, int n, double *__restrict__ partial) {
-  __shared__ X s_x[kColBlock];
-  const int64_t b = xcd_unit(blockIdx.x, nblocks);
-  if (b >= nblocks) return;
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  const int64_t c0 = b * kColBlock;
-  const int cn = (int)((int64_t)n - c0 < kColBlock ? (int64_t)n - c0 : kColBlock);
-  for (int k = threadIdx.x; k < cn; k += kBlock) s_x[k] = x[c0 + k];
-  __syncthreads();
-  for (int r = wid; r < nlong; r += kBlock / kWave) {
-    const int s = seg[b * nlong + r], e = seg[(b + 1) * nlong + r];
-    double sum = 0.0, sum2 = 0.0;
-    int j = s + lane;
-    for (; j + kWave < e; j += 2 * kWave) {
-      sum += prod_lds(lval[j], s_x, (int)(lcol[j] - c0));
-      sum2 += prod_lds(lval[j + kWave], s_x, (int)(lcol[j + kWave] - c0));
-    }
-    if (j < e) sum += prod_lds(lval[j], s_x, (int)(lcol[j] - c0));
-    sum += sum2;
+// upper_bound(s[0..n], key) over an LDS array
+__device__ inline int lds_upper(const int32_t *s, int n, int32_t key) {
+  int lo = 0, hi = n + 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (s[mid] <= key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct I4 {
+  int32_t a, b, c, d;
+};
+struct D2 {
+  double a, b;
+};
+struct F4 {
+  float a, b, c, d;
+};
+
+__device__ inline void load8(const int32_t *p, int32_t *o) {
+  const I4 u = *reinterpret_cast<const I4 *>(p), w = *reinterpret_cast<const I4 *>(p + 4);
+  o[0] = u.a; o[1] = u.b; o[2] = u.c; o[3] = u.d; o[4] = w.a; o[5] = w.b; o[6] = w.c; o[7] = w.d;
+}
+__device__ inline void load8(const double *p, double *o) {
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kWave);
-    if (lane == 0) partial[b * nlong + r] = sum;
+  for (int i = 0; i < 4; i++) {
+    const D2 u = reinterpret_cast<const D2 *>(p)[i];
+    o[2 * i] = u.a;
+    o[2 * i + 1] = u.b;
+  }
+}
+__device__ inline void load8(const float *p, float *o) {
+  const F4 u = *reinterpret_cast<const F4 *>(p), w = *reinterpret_cast<const F4 *>(p + 4);
+  o[0] = u.a; o[1] = u.b; o[2] = u.c; o[3] = u.d; o[4] = w.a; o[5] = w.b; o[6] = w.c; o[7] = w.d;
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kBlock) void k_long(const int32_t *__restrict__ brp, int nlong, int nblocks,
+                                                 const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                                                 const X *__restrict__ x, int n, double *__restrict__ partial) {
+  constexpr int LE = 8;  // nonzeros per thread per chunk
+  __shared__ X s_x[kColBlock];
+  __shared__ int32_t s_rp[kMaxLong + 1];
+  __shared__ double s_acc[kMaxLong];
+  __shared__ double s_wv[kBlock / kWave];
+  __shared__ int s_ws[kBlock / kWave];
+  const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+  for (int64_t iter = 0;; iter++) {
+    const int64_t b = next_unit(nullptr, iter, nblocks, nullptr);
+    if (b >= nblocks) break;
+    const int64_t c0 = b * kColBlock;
+    const int cn = (int)((int64_t)n - c0 < kColBlock ? (int64_t)n - c0 : kColBlock);
+    for (int k = tid; k < cn; k += kBlock) s_x[k] = x[c0 + k];
+    for (int r = tid; r <= nlong; r += kBlock) s_rp[r] = brp[b * nlong + r];
+    for (int r = tid; r < nlong; r += kBlock) s_acc[r] = 0.0;
+    __syncthreads();
+    const int E0 = s_rp[0], E1 = s_rp[nlong];
+    for (int c = E0 & ~(LE - 1); c < E1; c += kBlock * LE) {
+      const int e0 = c + tid * LE;
+      double p[LE];
+      {
+        int32_t ci[LE];
+        V vv[LE];
+        if (e0 < E1) {  // the arrays are padded, so a whole group of 8 is readable
+          load8(bcol + e0, ci);
+          load8(bval + e0, vv);
+        } else {
+#pragma unroll
+          for (int i = 0; i < LE; i++) {
+            ci[i] = (int32_t)c0;
+            vv[i] = (V)0;
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < LE; i++) {
+          const int e = e0 + i;
+          // nonzeros outside [E0, E1) belong to other blocks: never index LDS with them
+          const int lc = (e >= E0 && e < E1) ? (int)(ci[i] - c0) : 0;
+          p[i] = (e >= E0 && e < E1) ? prod_lds(vv[i], s_x, lc) : 0.0;
+        }
+      }
+      // walk this thread's nonzeros over the slice boundaries, starting from
+      // the slice of the previous nonzero: a slice that ends exactly at e0 is
+      // closed here, so the scan's carry-in lands on the slice it belongs to
+      int r = lds_upper(s_rp, nlong, e0 - 1) - 1;
+      if (r < 0) r = 0;
+      if (r > nlong - 1) r = nlong - 1;
+      double run = 0.0, first_val = 0.0;
+      int has_close = 0, first_r = 0;
+#pragma unroll
+      for (int i = 0; i < LE; i++) {
+        const int e = e0 + i;
+        if (e < E1) {
+          while (r < nlong - 1 && s_rp[r + 1] <= e) {
+            if (!has_close) {
+              first_val = run;
+              first_r = r;
+              has_close = 1;
+            } else {
+              s_acc[r] += run;  // piece opened and closed by this thread alone
+            }
+            run = 0.0;
+            r++;
+          }
+          run += p[i];
+        }
+      }
+      // segmented inclusive scan c(t) = run(t) + (has_close(t) ? 0 : c(t-1))
+      double v = run;
+      int sg = has_close;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const double vu = __shfl_up(v, off, kWave);
+        const int su = __shfl_up(sg, off, kWave);
+        if (lane >= off && !sg) {
+          v += vu;
+          sg |= su;
+        }
+      }
+      if (lane == kWave - 1) {
+        s_wv[wid] = v;
+        s_ws[wid] = sg;
+      }
+      __syncthreads();
+      double prev = 0.0;
+      for (int w = 0; w < wid; w++) prev = s_wv[w] + (s_ws[w] ? 0.0 : prev);
+      const double c_incl = v + (sg ? 0.0 : prev);
+      double carry_in = __shfl_up(c_incl, 1, kWave);
+      if (lane == 0) carry_in = prev;
+      // the first slice this thread closes got its earlier pieces from the scan
+      if (has_close) s_acc[first_r] += first_val + carry_in;
+      __syncthreads();
+      if (tid == kBlock - 1) {
+        // the slice still open at the end of the chunk takes the scan total
+        const int last = (c + kBlock * LE < E1 ? c + kBlock * LE : E1) - 1;
+        int rl = lds_upper(s_rp, nlong, last) - 1;
+        if (rl > nlong - 1) rl = nlong - 1;
+        if (rl >= 0) s_acc[rl] += c_incl;
+      }
+      __syncthreads();
+    }
+    for (int r = tid; r < nlong; r += kBlock) partial[b * nlong + r] = s_acc[r];
+    __syncthreads();
   }
 }
 
@@ -429,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V, X> a) {
     int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
     if (u >= a.unit_count) break;
     const Unit U = a.units[a.unit_begin + u];
-    const Part &q = a.parts[U.part];
+    const Part q = a.parts[U.part];  // by value: y stores cannot alias it
     const int64_t r = (int64_t)q.row_lo + (int64_t)U.local * G + g;
     if (r < q.row_hi) {
       int64_t b = a.rowptr[r];
@@ -481,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V, X> a) {
     int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
     if (u >= a.unit_count) break;
     const Unit U = a.units[a.unit_begin + u];
-    const Part &q = a.parts[U.part];
+    const Part q = a.parts[U.part];  // by value: y stores cannot alias it
     const Coord c0 = a.coords[q.coord_off + U.local];
     const Coord c1 = a.coords[q.coord_off + U.local + 1];
     const int i0 = c0.row, j0 = c0.nz;
@@ -613,7 +672,7 @@ __device__ inline void tile_emit(const KArgs<V, X> &a, const Part &q, int64_t lr
     a.y[r] = y_update(a.alpha, a.beta, v, a.y, r);
 }
 
-template <typename V, typename X>
+template <typename V, typename X, int MAXS>
 __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
   __shared__ int s_slot;
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -622,7 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
     int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
     if (u >= a.unit_count) break;
     const Unit U = a.units[a.unit_begin + u];
-    const Part &q = a.parts[U.part];
+    const Part q = a.parts[U.part];  // by value: y stores cannot alias it
     const int64_t t = (int64_t)U.local * (kBlock / kWave) + wid;
     if (t < q.ntiles) {
       const int64_t nz = q.e - q.s + 1;
@@ -634,6 +693,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
       int64_t tend = tstart + (int64_t)kOmega * sigma;
       if (tend > nz) tend = nz;
       const uint32_t fl = a.tile_flags[(q.tile_off + t) * kOmega + lane];
+      // does a row start exactly where this tile ends?  (bit 0 of the next
+      // tile's lane 0; a row open at the partition end does not continue)
+      const bool next_starts = tend >= nz || (a.tile_flags[(q.tile_off + t + 1) * kOmega] & 1u);
       // exclusive prefix count of row starts over lanes
       int excl = __popc(fl);
 #pragma unroll
@@ -648,19 +710,39 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
       double sum = 0.0, head = 0.0;
       int seen = 0;
       const int64_t lpos = tstart + (int64_t)lane * sigma;
-      for (int k = 0; k < sigma; k++) {
-        if ((fl >> k) & 1u) {
-          if (seen)
-            tile_emit(a, q, cur, sum);
-          else
-            head = sum;
-          seen = 1;
-          sum = 0.0;
-          cur = dirty ? local_row_search(a.rowptr, q, R0, R1 + 1, lpos + k) : cur + 1;
+      // all column/value loads first, then all x gathers, then the
+      // flag-driven segmented sum out of registers (the tile storage is
+      // padded to whole tiles, so loads need no bound; products past nz do)
+      double p[MAXS];
+      {
+        int c[MAXS];
+        V v[MAXS];
+#pragma unroll
+        for (int k = 0; k < MAXS; k++) {
+          c[k] = 0;
+          v[k] = (V)0;
+          if (k < sigma) {
+            const int64_t idx = base + (int64_t)k * kOmega + lane;
+            c[k] = a.col[idx];
+            v[k] = a.val[idx];
+          }
         }
-        if (lpos + k < nz) {
-          const int64_t idx = base + (int64_t)k * kOmega + lane;
-          sum += prod(a.val[idx], a.x, a.col[idx]);
+#pragma unroll
+        for (int k = 0; k < MAXS; k++) p[k] = (k < sigma && lpos + k < nz) ? prod(v[k], a.x, c[k]) : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < MAXS; k++) {
+        if (k < sigma) {
+          if ((fl >> k) & 1u) {
+            if (seen)
+              tile_emit(a, q, cur, sum);
+            else
+              head = sum;
+            seen = 1;
+            sum = 0.0;
+            cur = dirty ? local_row_search(a.rowptr, q, R0, R1 + 1, lpos + k) : cur + 1;
+          }
+          sum += p[k];
         }
       }
       double tail = 0.0;
@@ -689,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
         const double tot = tail + tnext;
         if (!later) {
           // the last row started in this tile: carried on if it runs past the tile
-          if (lrp(a.rowptr, q, cur + 1) > tend) {
+          if (!next_starts) {
             put_carry(a.crow, a.cval, slot0 + 1, (int)(q.row_lo + cur), tot);
           } else {
             tile_emit(a, q, cur, tot);
@@ -792,28 +874,6 @@ hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow, con
   return hipGetLastError();
 }
 
-hipError_t launch_long_seg(const int32_t *lrowptr, const int32_t *lcol, int nlong, int nblocks, int32_t *seg,
-                           hipStream_t st) {
-  const int64_t tot = (int64_t)(nblocks + 1) * nlong;
-  if (tot <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_long_seg, dim3(nblk(tot)), dim3(kBlock), 0, st, lrowptr, lcol, nlong, nblocks, seg);
-  return hipGetLastError();
-}
-
-hipError_t launch_long_f64(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol, const double *lval,
-                           const double *x, int n, double *partial, hipStream_t st) {
-  hipLaunchKernelGGL((k_long<double, double>), dim3(direct_grid(nblocks)), dim3(kBlock), 0, st, seg, nlong, nblocks,
-                     lcol, lval, x, n, partial);
-  return hipGetLastError();
-}
-
-hipError_t launch_long_f32(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol, const float *lval,
-                           const XPair *x, int n, double *partial, hipStream_t st) {
-  hipLaunchKernelGGL((k_long<float, XPair>), dim3(direct_grid(nblocks)), dim3(kBlock), 0, st, seg, nlong, nblocks,
-                     lcol, lval, x, n, partial);
-  return hipGetLastError();
-}
-
 hipError_t launch_long_reduce(const double *partial, int nlong, int nblocks, const int32_t *rows, double alpha,
                               double *y, hipStream_t st) {
   hipLaunchKernelGGL(k_long_reduce, dim3(nblk((int64_t)nlong * kWave)), dim3(kBlock), 0, st, partial, nlong, nblocks,
@@ -826,16 +886,54 @@ hipError_t launch_reset(int *q, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Workgroups that fit resident on the whole device for kernel `k`
+// (occupancy x CUs), rounded down to a multiple of kXcd.
+static int resident_cap(const void *k) {
+  int nb = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  int cap = nb * cus / kXcd * kXcd;
+  return cap < kXcd ? kXcd : cap;
+}
+
+// Grid for a launch: at most what fits resident, at least one workgroup per
+// XCD, always a multiple of kXcd (the unit walk divides by it).
+static int launch_grid(int want, int cap) {
+  int g = want < cap ? want : cap;
+  g = g / kXcd * kXcd;
+  return g < kXcd ? kXcd : g;
+}
+
+#define RESIDENT(kern) ([] { static const int c = resident_cap((const void *)(kern)); return c; }())
+
+hipError_t launch_long_f64(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol, const double *bval,
+                           const double *x, int n, double *partial, hipStream_t st) {
+  if (nblocks <= 0 || nlong <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_long<double, double>), dim3(launch_grid(direct_grid(nblocks), RESIDENT((k_long<double, double>)))),
+                     dim3(kBlock), 0, st, brp, nlong, nblocks, bcol, bval, x, n, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_long_f32(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol, const float *bval,
+                           const XPair *x, int n, double *partial, hipStream_t st) {
+  if (nblocks <= 0 || nlong <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_long<float, XPair>), dim3(launch_grid(direct_grid(nblocks), RESIDENT((k_long<float, XPair>)))),
+                     dim3(kBlock), 0, st, brp, nlong, nblocks, bcol, bval, x, n, partial);
+  return hipGetLastError();
+}
+
 template <typename V, typename X>
 static hipError_t vec_dispatch(const KArgs<V, X> &a, int lanes, int grid, hipStream_t st) {
   switch (lanes) {
-    case 1: hipLaunchKernelGGL((k_vector<V, X, 1>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_vector<V, X, 2>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_vector<V, X, 4>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_vector<V, X, 8>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((k_vector<V, X, 16>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 32: hipLaunchKernelGGL((k_vector<V, X, 32>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 64: hipLaunchKernelGGL((k_vector<V, X, 64>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_vector<V, X, 1>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 1>)))), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_vector<V, X, 2>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 2>)))), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_vector<V, X, 4>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 4>)))), dim3(kBlock), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_vector<V, X, 8>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 8>)))), dim3(kBlock), 0, st, a); break;
+    case 16: hipLaunchKernelGGL((k_vector<V, X, 16>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 16>)))), dim3(kBlock), 0, st, a); break;
+    case 32: hipLaunchKernelGGL((k_vector<V, X, 32>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 32>)))), dim3(kBlock), 0, st, a); break;
+    case 64: hipLaunchKernelGGL((k_vector<V, X, 64>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 64>)))), dim3(kBlock), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -844,12 +942,12 @@ static hipError_t vec_dispatch(const KArgs<V, X> &a, int lanes, int grid, hipStr
 template <typename V, typename X>
 static hipError_t merge_dispatch(const KArgs<V, X> &a, int items, int grid, hipStream_t st) {
   switch (items) {
-    case 2: hipLaunchKernelGGL((k_merge<V, X, 2>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_merge<V, X, 4>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_merge<V, X, 6>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_merge<V, X, 8>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 12: hipLaunchKernelGGL((k_merge<V, X, 12>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((k_merge<V, X, 16>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_merge<V, X, 2>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 2>)))), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_merge<V, X, 4>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 4>)))), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_merge<V, X, 6>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 6>)))), dim3(kBlock), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_merge<V, X, 8>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 8>)))), dim3(kBlock), 0, st, a); break;
+    case 12: hipLaunchKernelGGL((k_merge<V, X, 12>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 12>)))), dim3(kBlock), 0, st, a); break;
+    case 16: hipLaunchKernelGGL((k_merge<V, X, 16>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 16>)))), dim3(kBlock), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -867,13 +965,21 @@ hipError_t launch_merge_f64(const KArgs<double, double> &a, int items, int grid,
 hipError_t launch_merge_f32(const KArgs<float, XPair> &a, int items, int grid, hipStream_t st) {
   return merge_dispatch(a, items, grid, st);
 }
-hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_tile<double, double>), dim3(grid), dim3(kBlock), 0, st, a);
+template <typename V, typename X>
+static hipError_t tile_dispatch(const KArgs<V, X> &a, int grid, hipStream_t st) {
+  if (a.sigma <= 8)
+    hipLaunchKernelGGL((k_tile<V, X, 8>), dim3(launch_grid(grid, RESIDENT((k_tile<V, X, 8>)))), dim3(kBlock), 0, st, a);
+  else if (a.sigma <= 16)
+    hipLaunchKernelGGL((k_tile<V, X, 16>), dim3(launch_grid(grid, RESIDENT((k_tile<V, X, 16>)))), dim3(kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_tile<V, X, 32>), dim3(launch_grid(grid, RESIDENT((k_tile<V, X, 32>)))), dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
+hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st) {
+  return tile_dispatch(a, grid, st);
+}
 hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_tile<float, XPair>), dim3(grid), dim3(kBlock), 0, st, a);
-  return hipGetLastError();
+  return tile_dispatch(a, grid, st);
 }
 
 }  // namespace spx

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -97,9 +98,11 @@ struct spx_plan {
   // long rows
   int nlong = 0, nblocks = 0;
   int64_t long_nnz = 0, long_th = 0;
-  int32_t *d_lrowptr = nullptr, *d_lcol = nullptr, *d_lrows = nullptr, *d_seg = nullptr;
-  double *d_lval = nullptr, *d_partial = nullptr;
-  float *d_lval32 = nullptr;
+  int32_t *d_lrows = nullptr;
+  double *d_partial = nullptr;
+  int32_t *d_brp = nullptr, *d_bcol = nullptr;  // block-major CSR of the long rows
+  double *d_bval = nullptr;
+  float *d_bval32 = nullptr;
   hipStream_t lstream = nullptr;
   hipEvent_t lfork = nullptr, ldone = nullptr;
 
@@ -118,8 +121,8 @@ struct spx_plan {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
-                    (void *)d_lrowptr, (void *)d_lcol, (void *)d_lrows, (void *)d_seg, (void *)d_lval,
-                    (void *)d_partial, (void *)d_lval32})
+                    (void *)d_lrows, (void *)d_partial, (void *)d_brp, (void *)d_bcol, (void *)d_bval,
+                    (void *)d_bval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
     for (auto e : done) (void)hipEventDestroy(e);
@@ -179,6 +182,39 @@ static int64_t long_threshold(const spx_plan_desc *d) {
   return std::max<int64_t>(4096, 4 * blocks);
 }
 
+// The long rows as a block-major CSR: row (b, r) = the nonzeros of long row r
+// whose columns fall in column block b, ordered by block first.  Successive
+// units of any kernel then gather x from one kColBlock-wide window, which
+// stays in L2; the plan computes partial[b * nlong + r] into d_partial.
+static void long_block_csr(const std::vector<int32_t> &lrp, const std::vector<int32_t> &lcol,
+                           const std::vector<double> &lval, int nlong, int nblocks, std::vector<int64_t> &brp,
+                           std::vector<int32_t> &bcol, std::vector<double> &bval) {
+  std::vector<int64_t> cut((size_t)nlong * (nblocks + 1));  // cut[r*(nblocks+1)+b]: first element of r in block b
+  for (int r = 0; r < nlong; r++) {
+    int64_t j = lrp[r];
+    for (int b = 0; b <= nblocks; b++) {
+      const int64_t lo = (int64_t)b * kColBlock;
+      while (j < lrp[r + 1] && lcol[j] < lo) j++;
+      cut[(size_t)r * (nblocks + 1) + b] = b == nblocks ? lrp[r + 1] : j;
+    }
+  }
+  brp.assign((size_t)nblocks * nlong + 1, 0);
+  for (int b = 0; b < nblocks; b++)
+    for (int r = 0; r < nlong; r++) {
+      const size_t k = (size_t)b * nlong + r;
+      brp[k + 1] = brp[k] + (cut[(size_t)r * (nblocks + 1) + b + 1] - cut[(size_t)r * (nblocks + 1) + b]);
+    }
+  bcol.resize(lcol.size());
+  bval.resize(lval.size());
+  for (int b = 0; b < nblocks; b++)
+    for (int r = 0; r < nlong; r++) {
+      const int64_t s0 = cut[(size_t)r * (nblocks + 1) + b], s1 = cut[(size_t)r * (nblocks + 1) + b + 1];
+      const int64_t d0 = brp[(size_t)b * nlong + r];
+      std::copy(lcol.begin() + s0, lcol.begin() + s1, bcol.begin() + d0);
+      std::copy(lval.begin() + s0, lval.begin() + s1, bval.begin() + d0);
+    }
+}
+
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int m = d->m, n = d->n;
   const int64_t nnz = d->nnz;
@@ -227,9 +263,19 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   // ---- split off long rows ----
   P->long_th = long_threshold(d);
   std::vector<int32_t> long_rows;
-  if (P->long_th > 0)
+  if (P->long_th > 0) {
+    // at most kMaxLong long rows (the longest ones): raise the threshold if needed
+    std::vector<int32_t> lens;
+    for (int i = 0; i < m; i++)
+      if (rp[i + 1] - rp[i] >= P->long_th) lens.push_back(rp[i + 1] - rp[i]);
+    if ((int)lens.size() > kMaxLong) {
+      std::nth_element(lens.begin(), lens.begin() + kMaxLong, lens.end(), std::greater<int32_t>());
+      P->long_th = (int64_t)lens[kMaxLong] + 1;
+    }
     for (int i = 0; i < m; i++)
       if (rp[i + 1] - rp[i] >= P->long_th) long_rows.push_back(i);
+    if ((int)long_rows.size() > kMaxLong) long_rows.resize(kMaxLong);  // ties at the cut
+  }
   P->nlong = (int)long_rows.size();
 
   const int32_t *src_col = d->col;
@@ -455,21 +501,28 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   }
   if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
 
-  // ---- long-row payload ----
+  // ---- long-row payload: block-major CSR swept by k_long ----
   if (P->nlong > 0) {
     P->nblocks = (int)(((int64_t)n + kColBlock - 1) / kColBlock);
-    if ((rc = P->upload(&P->d_lrowptr, lrp.data(), P->nlong + 1))) return rc;
-    if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
-    if ((rc = P->upload(&P->d_lcol, lcol.data(), P->long_nnz))) return rc;
+    if ((int64_t)P->nblocks * P->nlong >= INT32_MAX) return fail(SPX_ERANGE, "too many long-row blocks");
+    std::vector<int64_t> brp;
+    std::vector<int32_t> bcol;
+    std::vector<double> bval;
+    long_block_csr(lrp, lcol, lval, P->nlong, P->nblocks, brp, bcol, bval);
+    // padded so k_long can read whole groups of 8 past the last nonzero
+    bcol.resize(P->long_nnz + kLongPad, 0);
+    bval.resize(P->long_nnz + kLongPad, 0.0);
+    std::vector<int32_t> brp32(brp.begin(), brp.end());
+    if ((rc = P->upload(&P->d_brp, brp32.data(), (int64_t)brp32.size()))) return rc;
+    if ((rc = P->upload(&P->d_bcol, bcol.data(), P->long_nnz + kLongPad))) return rc;
     if (P->precision == SPX_PREC_F32) {
-      std::vector<float> l32(lval.begin(), lval.end());
-      if ((rc = P->upload(&P->d_lval32, l32.data(), P->long_nnz))) return rc;
+      std::vector<float> b32(bval.begin(), bval.end());
+      if ((rc = P->upload(&P->d_bval32, b32.data(), P->long_nnz + kLongPad))) return rc;
     } else {
-      if ((rc = P->upload(&P->d_lval, lval.data(), P->long_nnz))) return rc;
+      if ((rc = P->upload(&P->d_bval, bval.data(), P->long_nnz + kLongPad))) return rc;
     }
-    if ((rc = P->alloc(&P->d_seg, (int64_t)(P->nblocks + 1) * P->nlong))) return rc;
+    if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
     if ((rc = P->alloc(&P->d_partial, (int64_t)P->nblocks * P->nlong))) return rc;
-    HIP_TRY(launch_long_seg(P->d_lrowptr, P->d_lcol, P->nlong, P->nblocks, P->d_seg, 0));
     HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&P->ldone, hipEventDisableTiming));
@@ -651,10 +704,10 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
     HIP_TRY(hipEventRecord(P->lfork, st));
     HIP_TRY(hipStreamWaitEvent(P->lstream, P->lfork, 0));
     if (f32)
-      HIP_TRY(launch_long_f32(P->d_seg, P->nlong, P->nblocks, P->d_lcol, P->d_lval32, P->d_xhl, P->n, P->d_partial,
+      HIP_TRY(launch_long_f32(P->d_brp, P->nlong, P->nblocks, P->d_bcol, P->d_bval32, P->d_xhl, P->n, P->d_partial,
                               P->lstream));
     else
-      HIP_TRY(launch_long_f64(P->d_seg, P->nlong, P->nblocks, P->d_lcol, P->d_lval, x, P->n, P->d_partial,
+      HIP_TRY(launch_long_f64(P->d_brp, P->nlong, P->nblocks, P->d_bcol, P->d_bval, x, P->n, P->d_partial,
                               P->lstream));
     HIP_TRY(hipEventRecord(P->ldone, P->lstream));
   }
@@ -820,3 +873,118 @@ int spx_device_count(void) {
 }
 
 }  // extern "C"
+
+// ---- batched form: block-diagonal packing ------------------------------------------------
+
+extern "C" int spx_batch_create(spx_plan_t *out, int count, const spx_batch_item *items, const spx_plan_desc *tmpl,
+                                int64_t *x_off, int64_t *y_off) {
+  if (!out || count < 0 || (count > 0 && !items) || !tmpl) return fail(SPX_EARG, "bad batch arguments");
+  int64_t M = 0, N = 0, NNZ = 0;
+  for (int i = 0; i < count; i++) {
+    const spx_batch_item &it = items[i];
+    if (it.m < 0 || it.n < 0 || it.nnz < 0 || (it.m > 0 && !it.rowptr) || (it.nnz > 0 && (!it.col || !it.val)))
+      return fail(SPX_EARG, "bad batch item");
+    if (x_off) x_off[i] = N;
+    if (y_off) y_off[i] = M;
+    M += it.m;
+    N += it.n;
+    NNZ += it.nnz;
+  }
+  if (x_off) x_off[count] = N;
+  if (y_off) y_off[count] = M;
+  if (M >= INT32_MAX || N >= INT32_MAX) return fail(SPX_ERANGE, "batch too large: sum of m or n must be below 2^31");
+  if (NNZ >= ((int64_t)1 << 31) - 1) return fail(SPX_ERANGE, "batch too large: total nnz must be below 2^31");
+  std::vector<int64_t> rp((size_t)M + 1, 0);
+  std::vector<int32_t> col((size_t)NNZ);
+  std::vector<double> val((size_t)NNZ);
+  int64_t r0 = 0, c0 = 0, z0 = 0;
+  for (int i = 0; i < count; i++) {
+    const spx_batch_item &it = items[i];
+    if (it.m > 0 && (it.rowptr[0] != 0 || it.rowptr[it.m] != it.nnz)) return fail(SPX_EARG, "bad batch row pointer");
+    for (int r = 1; r <= it.m; r++) rp[r0 + r] = z0 + it.rowptr[r];
+    for (int64_t k = 0; k < it.nnz; k++) {
+      if (it.col[k] < 0 || it.col[k] >= it.n) return fail(SPX_EARG, "batch column index out of range");
+      col[z0 + k] = (int32_t)(it.col[k] + c0);
+    }
+    std::memcpy(val.data() + z0, it.val, sizeof(double) * (size_t)it.nnz);
+    r0 += it.m;
+    c0 += it.n;
+    z0 += it.nnz;
+  }
+  spx_plan_desc d = *tmpl;
+  d.m = (int)M;
+  d.n = (int)N;
+  d.nnz = NNZ;
+  d.rowptr64 = rp.data();
+  d.rowptr32 = nullptr;
+  d.col = col.data();
+  d.val = val.data();
+  d.flags &= ~SPX_FLAG_DEVICE_PTRS;
+  return spx_plan_create(out, &d);
+}
+
+// ---- HIP graphs ----------------------------------------------------------------------------
+
+struct spx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~spx_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+extern "C" int spx_graph_capture(spx_plan_t p, double alpha, const double *x, double beta, double *y,
+                                 spx_graph_t *out) {
+  if (!p || !out) return fail(SPX_EARG, "null argument");
+  *out = nullptr;
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::unique_ptr<spx_graph> G(new spx_graph());
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  int rc = SPX_OK;
+  if (e == hipSuccess) {
+    rc = execute_impl(p, alpha, x, beta, y, st, nullptr, nullptr, false);
+    hipError_t e2 = hipStreamEndCapture(st, &G->graph);
+    if (rc == SPX_OK && e2 != hipSuccess) e = e2;
+  }
+  if (rc == SPX_OK && e == hipSuccess) e = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(st);
+  if (rc != SPX_OK) return rc;
+  HIP_TRY(e);
+  *out = G.release();
+  return SPX_OK;
+}
+
+extern "C" int spx_graph_launch(spx_graph_t g, void *stream) {
+  if (!g) return fail(SPX_EARG, "null graph");
+  HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return SPX_OK;
+}
+
+extern "C" int spx_graph_destroy(spx_graph_t g) {
+  delete g;
+  return SPX_OK;
+}
+
+extern "C" int spx_graph_time(spx_graph_t g, int warmup, int reps, double *out_ms) {
+  if (!g || reps <= 0 || !out_ms) return fail(SPX_EARG, "bad timing arguments");
+  hipStream_t st = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipError_t e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  for (int i = 0; i < warmup && e == hipSuccess; i++) e = hipGraphLaunch(g->exec, st);
+  if (e == hipSuccess) e = hipEventRecord(a, st);
+  for (int i = 0; i < reps && e == hipSuccess; i++) e = hipGraphLaunch(g->exec, st);
+  if (e == hipSuccess) e = hipEventRecord(b, st);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  (void)hipStreamDestroy(st);
+  HIP_TRY(e);
+  out_ms[0] = ms / reps;
+  return SPX_OK;
+}

@@ -13,7 +13,9 @@ constexpr int kBlock = 256;   // threads per workgroup (4 wave64s)
 constexpr int kWave = 64;     // CDNA wavefront width
 constexpr int kOmega = 64;    // lanes per CSR5-style tile
 constexpr int kXcd = 8;       // accelerator complex dies (one L2 each)
-constexpr int kColBlock = 4096;  // long-row path: x entries staged in LDS per block
+constexpr int kColBlock = 2048;   // long-row path: x entries staged in LDS per column block
+constexpr int kLongPad = 16;      // long-row path: padding of the block-major arrays
+constexpr int kMaxLong = 1024;    // long-row path: most long rows per plan
 
 // Workgroups for a direct launch over `count` units: a multiple of kXcd so
 // every XCD receives the same contiguous share.
@@ -105,13 +107,11 @@ hipError_t launch_empty_rows(const int32_t *rows, int64_t nrows, double beta,
 hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow,
                         const double *cval, double alpha, double beta, double *y,
                         hipStream_t st);
-hipError_t launch_long_seg(const int32_t *lrowptr, const int32_t *lcol, int nlong,
-                           int nblocks, int32_t *seg, hipStream_t st);
-hipError_t launch_long_f64(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol,
-                           const double *lval, const double *x, int n, double *partial,
+hipError_t launch_long_f64(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol,
+                           const double *bval, const double *x, int n, double *partial,
                            hipStream_t st);
-hipError_t launch_long_f32(const int32_t *seg, int nlong, int nblocks, const int32_t *lcol,
-                           const float *lval, const XPair *x, int n, double *partial,
+hipError_t launch_long_f32(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol,
+                           const float *bval, const XPair *x, int n, double *partial,
                            hipStream_t st);
 hipError_t launch_long_reduce(const double *partial, int nlong, int nblocks,
                               const int32_t *rows, double alpha, double *y, hipStream_t st);

@@ -69,7 +69,17 @@ CAPI = {
     "spx_device_count": (_I, []),
     "spx_last_error": (C.c_char_p, []),
     "spx_version": (C.c_char_p, []),
+    "spx_batch_create": (_I, [_P, _I, _P, _P, _P, _P]),
+    "spx_graph_capture": (_I, [_P, _D, _P, _D, _P, _P]),
+    "spx_graph_launch": (_I, [_P, _P]),
+    "spx_graph_destroy": (_I, [_P]),
+    "spx_graph_time": (_I, [_P, _I, _I, _P]),
 }
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("nnz", C.c_int64),
+                ("rowptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p)]
 
 # C++-linkage drop-in symbols of include/spmv_kernel.h (Itanium mangling)
 CXX_SYMBOLS = {
@@ -233,6 +243,64 @@ class Plan:
     def close(self):
         if getattr(self, "h", None):
             lib().spx_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchPlan(Plan):
+    """Many independent SpMVs as one block-diagonal plan (spx_batch_create).
+    x_off / y_off give each matrix's slice of the concatenated x and y."""
+
+    def __init__(self, mats, kernel=KERNEL_MERGE, mode=MODE_QUEUE, parts=1, nb=1 << 22, queues=1, sigma=0,
+                 lanes=0, items=0, precision=PREC_F64, long_row=0):
+        keep = []
+        arr = (BatchItem * len(mats))()
+        for i, A in enumerate(mats):
+            rp = np.ascontiguousarray(A["rowptr"], np.int64)
+            c = np.ascontiguousarray(A["col"], np.int32)
+            v = np.ascontiguousarray(A["val"], np.float64)
+            keep += [rp, c, v]
+            arr[i] = BatchItem(A["m"], A["n"], int(rp[-1]), rp.ctypes.data, c.ctypes.data if len(c) else None,
+                               v.ctypes.data if len(v) else None)
+        d = PlanDesc()
+        lib().spx_plan_desc_init(C.byref(d))
+        d.kernel, d.mode, d.parts, d.nb, d.queues = kernel, mode, parts, nb, queues
+        d.sigma, d.lanes, d.items, d.precision, d.long_row = sigma, lanes, items, precision, long_row
+        self.x_off = np.zeros(len(mats) + 1, np.int64)
+        self.y_off = np.zeros(len(mats) + 1, np.int64)
+        h = C.c_void_p()
+        check(lib().spx_batch_create(C.byref(h), len(mats), arr, C.byref(d), self.x_off.ctypes.data,
+                                     self.y_off.ctypes.data))
+        self.h = h
+        self.m, self.n = int(self.y_off[-1]), int(self.x_off[-1])
+        self.nnz = sum(int(A["rowptr"][-1]) for A in mats)
+
+
+class Graph:
+    """An instantiated hipGraph of one plan execute (spx_graph_capture)."""
+
+    def __init__(self, plan, x_dev, y_dev, alpha=1.0, beta=0.0):
+        h = C.c_void_p()
+        check(lib().spx_graph_capture(plan.h, alpha, x_dev.ptr, beta, y_dev.ptr, C.byref(h)))
+        self.h = h
+        self._keep = (plan, x_dev, y_dev)
+
+    def launch(self, stream=None):
+        check(lib().spx_graph_launch(self.h, stream))
+
+    def time(self, warmup=3, reps=20):
+        out = (C.c_double * 1)()
+        check(lib().spx_graph_time(self.h, warmup, reps, out))
+        return out[0]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().spx_graph_destroy(self.h)
             self.h = None
 
     def __del__(self):

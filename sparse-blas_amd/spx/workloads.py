@@ -61,7 +61,7 @@ def zipf(n=8192, s=1.5, kmax=512, seed=1, value_law=None):
     return build(ZIPF, n, n, seed, [s, 1, kmax], value_law)
 
 
-def circuit(m=5_558_326, seed=2, total=59_500_000, n_dense=200, window=32768, value_law=None):
+def circuit(m=5_558_326, seed=2, total=59_500_000, n_dense=200, window=1024, value_law=None):
     """Circuit-like skewed matrix.  Regular rows: k ~ Geometric(0.1) truncated
     to [1, 12] (realized mean ~5.3, not the stated 10), columns distinct in a
     +-window band around the diagonal (window=0: uniform over [0, n)).
@@ -89,7 +89,7 @@ def rail(m=4284, n=1_092_610, total=11_300_000, sigma=1.2, seed=3, value_law=VAL
     return build(LOGNORMAL, m, n, seed, [mu, sigma, 1, n], value_law)
 
 
-def asic(m=682_862, seed=4, window=32768, value_law=None):
+def asic(m=682_862, seed=4, window=1024, value_law=None):
     """ASIC-like: regular rows geometric (p=0.31, [1, 64], mean ~3.2), columns
     banded like `circuit`; 8 heavy rows log-uniform [1e4, 3.95e5], one pinned
     at 395k, uniform columns; values +-lognormal(0, 2).  About 3.9M nnz."""
@@ -132,6 +132,22 @@ def g_matrix(n, seed=1):
     rowptr = np.zeros(n + 1, np.int64)
     np.cumsum(counts, out=rowptr[1:])
     return finish(rowptr, c, v, n, n)
+
+
+def batch_sizes(count=512, seed=5, nmin=1000, nmax=200_000, mu=math.log(14142), sigma=1.32):
+    """Square sizes n_i ~ round(lognormal(mu, sigma)) clipped to [nmin, nmax]
+    (numpy PCG64 seeded; the law's parameters are not stated, these give a
+    median of ~14k and a total of ~1.2e9 nonzeros for 512 matrices)."""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.round(rng.lognormal(mu, sigma, count)), nmin, nmax).astype(np.int64)
+
+
+def batch(count=512, seed=5, s=1.3, kmax=2048, value_law=None, **size_kw):
+    """Independent matrices for the batched form: A_i is n_i-by-n_i with
+    k ~ Zipf(s) truncated to [1, min(kmax, n_i)] nonzeros per row at uniform
+    distinct columns, values U[-1,1).  Member i uses seed 1000*seed + i."""
+    ns = batch_sizes(count, seed, **size_kw)
+    return [zipf(int(n), s, min(kmax, int(n)), seed=1000 * seed + i, value_law=value_law) for i, n in enumerate(ns)]
 
 
 def bytes_model(m, n, nnz, value_bytes=8, beta_nonzero=False, k=1):
