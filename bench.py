@@ -46,6 +46,9 @@ def main():
                     help="half-width of the regular rows' column band (0 = uniform columns)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the timed region, one device-to-device copy of this many MiB: the "
+                         "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build drives one MI355X; --gpus must be 1 (the multi-GPU "
@@ -83,6 +86,15 @@ def main():
     wall_ms = (time.perf_counter() - t_start) * 1e3 / a.steps
     log(f"timed {a.steps} steps: wall {wall_ms:.4f} ms/step, events {tot_ms:.4f} ms/execute, "
         f"kernel {ker_ms:.4f} ms")
+
+    if a.copy_mb > 0:
+        nb = a.copy_mb << 20
+        src, dst = spx.DeviceBuffer(nb), spx.DeviceBuffer(nb)
+        spx.check(spx.lib().spx_memcpy_d2d(dst.ptr, src.ptr, nb))
+        spx.synchronize()
+        log(f"calibration copy: {a.copy_mb} MiB device to device")
+        src.free()
+        dst.free()
 
     B = W.bytes_model(m, n, nnz, 8, beta != 0.0)
     y = yd.download(np.float64, m)

@@ -43,7 +43,9 @@ enum {
 enum {
   SPX_KERNEL_VECTOR = 1, /* wavefront/lane-group per row CSR-vector */
   SPX_KERNEL_MERGE = 2,  /* merge-path over (rows + nonzeros) */
-  SPX_KERNEL_TILE = 3    /* CSR5-style 64-lane tiles with bit-flag segments */
+  SPX_KERNEL_TILE = 3,   /* CSR5-style 64-lane tiles with bit-flag segments */
+  SPX_KERNEL_SEG = 4     /* plan API only: row-aligned units of 8-aligned
+                            nonzeros, 16-byte loads, no carries */
 };
 
 /* Value storage. */
@@ -67,7 +69,8 @@ typedef struct spx_plan_desc {
   int queues;               /* QUEUE: streams (copy_of_workspace), >= 1 */
   int sigma;                /* TILE: elements per lane, 4..32 (0 = auto) */
   int lanes;                /* VECTOR: lanes per row, power of 2 <= 64 (0 = auto) */
-  int items;                /* MERGE: merge items per thread, 1..16 (0 = auto) */
+  int items;                /* MERGE: merge items per thread, 2..16 (0 = auto);
+                               SEG: chunks of 2048 nonzeros per unit, 1..32 */
   int precision;            /* SPX_PREC_* */
   int flags;                /* SPX_FLAG_* bits */
   int64_t long_row;         /* rows with at least this many nonzeros take the

@@ -292,186 +292,31 @@ __global__ void k_fixup(const Run *__restrict__ runs, int64_t nruns, const int32
   y[R.row] = y_update(alpha, beta, s, y, R.row);
 }
 
-// ---- long rows: column blocks with x staged in LDS --------------------------------
-// Rows with very many nonzeros at scattered columns make every x gather an
-// L2 request.  They are kept apart in a block-major CSR: slice (b, r) holds
-// the nonzeros of long row r whose columns fall in column block b, slices
-// ordered by block first (arrays padded by kLongPad entries).  One
-// workgroup per column block stages x[b*kColBlock, (b+1)*kColBlock) and the
-// block's slice pointers in LDS, then streams the block's nonzeros in
-// chunks of kBlock*8: each thread owns 8 consecutive, 8-aligned nonzeros
-// (16-byte vector loads), gathers x from LDS, walks the slice boundaries,
-// and a wave-shuffle segmented scan hands each slice piece to the thread that
-// closes it.  Pieces are added into an LDS accumulator per slice in a fixed
-// order, so the result is deterministic.  partial[b * nlong + r] is folded
-// per row by k_long_reduce.
-
-__device__ inline double prod_lds(double v, const double *sx, int c) { return v * sx[c]; }
-__device__ inline double prod_lds(float v, const XPair *sx, int c) { return prod(v, sx, c); }
-
-// upper_bound(s[0..n], key) over an LDS array
-__device__ inline int lds_upper(const int32_t *s, int n, int32_t key) {
-  int lo = 0, hi = n + 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (s[mid] <= key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-struct I4 {
-  int32_t a, b, c, d;
-};
-struct D2 {
-  double a, b;
-};
-struct F4 {
-  float a, b, c, d;
-};
-
-__device__ inline void load8(const int32_t *p, int32_t *o) {
-  const I4 u = *reinterpret_cast<const I4 *>(p), w = *reinterpret_cast<const I4 *>(p + 4);
-  o[0] = u.a; o[1] = u.b; o[2] = u.c; o[3] = u.d; o[4] = w.a; o[5] = w.b; o[6] = w.c; o[7] = w.d;
-}
-__device__ inline void load8(const double *p, double *o) {
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const D2 u = reinterpret_cast<const D2 *>(p)[i];
-    o[2 * i] = u.a;
-    o[2 * i + 1] = u.b;
-  }
-}
-__device__ inline void load8(const float *p, float *o) {
-  const F4 u = *reinterpret_cast<const F4 *>(p), w = *reinterpret_cast<const F4 *>(p + 4);
-  o[0] = u.a; o[1] = u.b; o[2] = u.c; o[3] = u.d; o[4] = w.a; o[5] = w.b; o[6] = w.c; o[7] = w.d;
-}
-
-template <typename V, typename X>
-__global__ __launch_bounds__(kBlock) void k_long(const int32_t *__restrict__ brp, int nlong, int nblocks,
-                                                 const int32_t *__restrict__ bcol, const V *__restrict__ bval,
-                                                 const X *__restrict__ x, int n, double *__restrict__ partial) {
-  constexpr int LE = 8;  // nonzeros per thread per chunk
-  __shared__ X s_x[kColBlock];
-  __shared__ int32_t s_rp[kMaxLong + 1];
-  __shared__ double s_acc[kMaxLong];
-  __shared__ double s_wv[kBlock / kWave];
-  __shared__ int s_ws[kBlock / kWave];
-  const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-  for (int64_t iter = 0;; iter++) {
-    const int64_t b = next_unit(nullptr, iter, nblocks, nullptr);
-    if (b >= nblocks) break;
-    const int64_t c0 = b * kColBlock;
-    const int cn = (int)((int64_t)n - c0 < kColBlock ? (int64_t)n - c0 : kColBlock);
-    for (int k = tid; k < cn; k += kBlock) s_x[k] = x[c0 + k];
-    for (int r = tid; r <= nlong; r += kBlock) s_rp[r] = brp[b * nlong + r];
-    for (int r = tid; r < nlong; r += kBlock) s_acc[r] = 0.0;
-    __syncthreads();
-    const int E0 = s_rp[0], E1 = s_rp[nlong];
-    for (int c = E0 & ~(LE - 1); c < E1; c += kBlock * LE) {
-      const int e0 = c + tid * LE;
-      double p[LE];
-      {
-        int32_t ci[LE];
-        V vv[LE];
-        if (e0 < E1) {  // the arrays are padded, so a whole group of 8 is readable
-          load8(bcol + e0, ci);
-          load8(bval + e0, vv);
-        } else {
-#pragma unroll
-          for (int i = 0; i < LE; i++) {
-            ci[i] = (int32_t)c0;
-            vv[i] = (V)0;
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < LE; i++) {
-          const int e = e0 + i;
-          // nonzeros outside [E0, E1) belong to other blocks: never index LDS with them
-          const int lc = (e >= E0 && e < E1) ? (int)(ci[i] - c0) : 0;
-          p[i] = (e >= E0 && e < E1) ? prod_lds(vv[i], s_x, lc) : 0.0;
-        }
-      }
-      // walk this thread's nonzeros over the slice boundaries, starting from
-      // the slice of the previous nonzero: a slice that ends exactly at e0 is
-      // closed here, so the scan's carry-in lands on the slice it belongs to
-      int r = lds_upper(s_rp, nlong, e0 - 1) - 1;
-      if (r < 0) r = 0;
-      if (r > nlong - 1) r = nlong - 1;
-      double run = 0.0, first_val = 0.0;
-      int has_close = 0, first_r = 0;
-#pragma unroll
-      for (int i = 0; i < LE; i++) {
-        const int e = e0 + i;
-        if (e < E1) {
-          while (r < nlong - 1 && s_rp[r + 1] <= e) {
-            if (!has_close) {
-              first_val = run;
-              first_r = r;
-              has_close = 1;
-            } else {
-              s_acc[r] += run;  // piece opened and closed by this thread alone
-            }
-            run = 0.0;
-            r++;
-          }
-          run += p[i];
-        }
-      }
-      // segmented inclusive scan c(t) = run(t) + (has_close(t) ? 0 : c(t-1))
-      double v = run;
-      int sg = has_close;
-#pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        const double vu = __shfl_up(v, off, kWave);
-        const int su = __shfl_up(sg, off, kWave);
-        if (lane >= off && !sg) {
-          v += vu;
-          sg |= su;
-        }
-      }
-      if (lane == kWave - 1) {
-        s_wv[wid] = v;
-        s_ws[wid] = sg;
-      }
-      __syncthreads();
-      double prev = 0.0;
-      for (int w = 0; w < wid; w++) prev = s_wv[w] + (s_ws[w] ? 0.0 : prev);
-      const double c_incl = v + (sg ? 0.0 : prev);
-      double carry_in = __shfl_up(c_incl, 1, kWave);
-      if (lane == 0) carry_in = prev;
-      // the first slice this thread closes got its earlier pieces from the scan
-      if (has_close) s_acc[first_r] += first_val + carry_in;
-      __syncthreads();
-      if (tid == kBlock - 1) {
-        // the slice still open at the end of the chunk takes the scan total
-        const int last = (c + kBlock * LE < E1 ? c + kBlock * LE : E1) - 1;
-        int rl = lds_upper(s_rp, nlong, last) - 1;
-        if (rl > nlong - 1) rl = nlong - 1;
-        if (rl >= 0) s_acc[rl] += c_incl;
-      }
-      __syncthreads();
-    }
-    for (int r = tid; r < nlong; r += kBlock) partial[b * nlong + r] = s_acc[r];
-    __syncthreads();
-  }
-}
-
-// One wave per long row: y[row] += alpha * sum over blocks (the regular pass
-// has already written beta*y0 for the row, which is empty there).
-__global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong, int nblocks,
+// ---- long rows: fold the per-column-block partial sums -----------------------
+// The segmented engine sweeps the long rows by column blocks and leaves one
+// partial sum per (column block, row) slice; lseg[lsp[r] .. lsp[r+1]) lists
+// row r's slices in column order.  One workgroup per long row adds them in a
+// fixed tree order: y[row] += alpha * sum (the regular pass has already
+// written beta * y0 for the row, which is empty there).
+__global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong,
+                                                        const int32_t *__restrict__ lsp,
+                                                        const int32_t *__restrict__ lseg,
                                                         const int32_t *__restrict__ rows, double alpha,
                                                         double *__restrict__ y) {
-  const int r = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-  const int lane = threadIdx.x % kWave;
+  __shared__ double s_w[kBlock / kWave];
+  const int r = blockIdx.x, tid = threadIdx.x;
   if (r >= nlong) return;
   double s = 0.0;
-  for (int b = lane; b < nblocks; b += kWave) s += partial[(int64_t)b * nlong + r];
+  for (int j = lsp[r] + tid; j < lsp[r + 1]; j += kBlock) s += partial[lseg[j]];
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-  if (lane == 0) y[rows[r]] += alpha * s;
+  if (tid % kWave == 0) s_w[tid / kWave] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; w++) t += s_w[w];
+    y[rows[r]] += alpha * t;
+  }
 }
 
 // ---- kernel 1: CSR-vector, L lanes per row --------------------------------------
@@ -874,10 +719,10 @@ hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow, con
   return hipGetLastError();
 }
 
-hipError_t launch_long_reduce(const double *partial, int nlong, int nblocks, const int32_t *rows, double alpha,
-                              double *y, hipStream_t st) {
-  hipLaunchKernelGGL(k_long_reduce, dim3(nblk((int64_t)nlong * kWave)), dim3(kBlock), 0, st, partial, nlong, nblocks,
-                     rows, alpha, y);
+hipError_t launch_long_reduce(const double *partial, int nlong, const int32_t *lsp, const int32_t *lseg,
+                              const int32_t *rows, double alpha, double *y, hipStream_t st) {
+  if (nlong <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_long_reduce, dim3(nlong), dim3(kBlock), 0, st, partial, nlong, lsp, lseg, rows, alpha, y);
   return hipGetLastError();
 }
 
@@ -907,22 +752,6 @@ static int launch_grid(int want, int cap) {
 }
 
 #define RESIDENT(kern) ([] { static const int c = resident_cap((const void *)(kern)); return c; }())
-
-hipError_t launch_long_f64(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol, const double *bval,
-                           const double *x, int n, double *partial, hipStream_t st) {
-  if (nblocks <= 0 || nlong <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_long<double, double>), dim3(launch_grid(direct_grid(nblocks), RESIDENT((k_long<double, double>)))),
-                     dim3(kBlock), 0, st, brp, nlong, nblocks, bcol, bval, x, n, partial);
-  return hipGetLastError();
-}
-
-hipError_t launch_long_f32(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol, const float *bval,
-                           const XPair *x, int n, double *partial, hipStream_t st) {
-  if (nblocks <= 0 || nlong <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_long<float, XPair>), dim3(launch_grid(direct_grid(nblocks), RESIDENT((k_long<float, XPair>)))),
-                     dim3(kBlock), 0, st, brp, nlong, nblocks, bcol, bval, x, n, partial);
-  return hipGetLastError();
-}
 
 template <typename V, typename X>
 static hipError_t vec_dispatch(const KArgs<V, X> &a, int lanes, int grid, hipStream_t st) {

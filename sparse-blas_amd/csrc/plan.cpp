@@ -1,8 +1,10 @@
 // plan.cpp -- device-resident SpMV plans behind the extern "C" ABI.
 //
 // spx_plan_create uploads A once and prepares everything the kernels need:
-//  - rows with very many nonzeros ("long rows") are split off into their own
-//    CSR, swept by column blocks with x staged in LDS (k_long);
+//  - rows with very many nonzeros ("long rows") are split off and cut into
+//    (column block, row) slices for the segmented engine, which stages each
+//    block's x in LDS; k_long_reduce folds the per-block sums;
+//  - kernel 4 hands the regular rows to the segmented engine too (one launch);
 //  - the remaining "regular" CSR is partitioned on the device (nnz-balanced
 //    partitions -> HIP streams; each partition -> workgroup units);
 //  - kernel-specific conversion: merge-path unit coordinates, or CSR5-style
@@ -99,11 +101,21 @@ struct spx_plan {
   int nlong = 0, nblocks = 0;
   int64_t long_nnz = 0, long_th = 0;
   int32_t *d_lrows = nullptr;
-  double *d_partial = nullptr;
-  int32_t *d_brp = nullptr, *d_bcol = nullptr;  // block-major CSR of the long rows
-  double *d_bval = nullptr;
-  float *d_bval32 = nullptr;
+  double *d_partial = nullptr;  // one sum per (column block, long row) slice
+  int32_t *d_lsp = nullptr, *d_lseg = nullptr;  // long row r's slices: lseg[lsp[r] .. lsp[r+1])
   hipStream_t lstream = nullptr;
+
+  // segmented engine: the long rows' (column block, row) slices for every
+  // kernel, plus the regular rows for kernel 4
+  SegUnit *d_sunits = nullptr;
+  int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work
+  uint8_t *d_sstart = nullptr;
+  int32_t *d_scol = nullptr;
+  double *d_sval = nullptr;
+  float *d_sval32 = nullptr;
+  int64_t nsunits = 0, nslong = 0;  // units; the first nslong sweep long rows
+  int64_t nlseg = 0;                 // long-row slices
+  bool sstage = false;
   hipEvent_t lfork = nullptr, ldone = nullptr;
 
   struct Launch {
@@ -121,8 +133,8 @@ struct spx_plan {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
-                    (void *)d_lrows, (void *)d_partial, (void *)d_brp, (void *)d_bcol, (void *)d_bval,
-                    (void *)d_bval32})
+                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_lsp, (void *)d_lseg,
+                    (void *)d_scol, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
     for (auto e : done) (void)hipEventDestroy(e);
@@ -182,13 +194,109 @@ static int64_t long_threshold(const spx_plan_desc *d) {
   return std::max<int64_t>(4096, 4 * blocks);
 }
 
-// The long rows as a block-major CSR: row (b, r) = the nonzeros of long row r
-// whose columns fall in column block b, ordered by block first.  Successive
-// units of any kernel then gather x from one kColBlock-wide window, which
-// stays in L2; the plan computes partial[b * nlong + r] into d_partial.
-static void long_block_csr(const std::vector<int32_t> &lrp, const std::vector<int32_t> &lcol,
-                           const std::vector<double> &lval, int nlong, int nblocks, std::vector<int64_t> &brp,
-                           std::vector<int32_t> &bcol, std::vector<double> &bval) {
+// Host side of the segmented engine: segments in output order, grouped into
+// units, nonzeros copied 8-aligned per unit, one start bit per element.
+struct SegHost {
+  std::vector<SegUnit> units;
+  std::vector<int32_t> out, col;
+  std::vector<double> val;
+  std::vector<uint8_t> start;
+  int64_t pos = 0;  // next free element
+  bool stage = false;
+};
+
+// Append segments [0, nseg) as units.  seg(i, b, e, o): segment i is source
+// elements [b, e) of scol/sval (never empty) with output o; brk(i): a unit
+// must not run on into segment i (the caller breaks where the outputs stop
+// being consecutive).  A unit takes whole segments while it
+// holds at most `target` elements (a longer segment sits alone).  With
+// `stage`, a unit whose column window fits kSegStage
+// entries stages x in LDS: the builder first grows the unit under that
+// window limit and keeps it staged if it reaches half the target (or the
+// last segment); otherwise the unit is built unstaged to the full target.
+template <typename F, typename G>
+static void seg_append(SegHost &H, int64_t nseg, F seg, G brk, const int32_t *scol, const double *sval,
+                       int64_t target, bool stage, int flags) {
+  auto window = [&](int64_t b, int64_t e, int32_t &lo, int32_t &hi) {
+    for (int64_t k = b; k < e; k++) {
+      lo = std::min(lo, scol[k]);
+      hi = std::max(hi, scol[k]);
+    }
+  };
+  int64_t i = 0;
+  while (i < nseg) {
+    // extent of a staged unit starting at segment i
+    int64_t j = i, el = 0;
+    bool staged = false;
+    if (stage) {
+      int32_t lo = INT32_MAX, hi = -1;
+      while (j < nseg) {
+        int64_t b, e;
+        int32_t o;
+        seg(j, b, e, o);
+        if (j > i && (el + (e - b) > target || brk(j))) break;
+        int32_t l2 = lo, h2 = hi;
+        window(b, e, l2, h2);
+        if ((int64_t)h2 - l2 + 1 > kSegStage) break;
+        lo = l2;
+        hi = h2;
+        el += e - b;
+        j++;
+      }
+      staged = j > i && (2 * el >= target || j == nseg);
+    }
+    if (!staged) {  // unstaged: whole segments up to the target
+      j = i;
+      el = 0;
+      while (j < nseg) {
+        int64_t b, e;
+        int32_t o;
+        seg(j, b, e, o);
+        if (j > i && (el + (e - b) > target || brk(j))) break;
+        el += e - b;
+        j++;
+      }
+    }
+    const int64_t E0 = (H.pos + 7) & ~(int64_t)7;
+    H.col.resize(E0, 0);
+    H.val.resize(E0, 0.0);
+    H.start.resize((size_t)((E0 + el + 7) >> 3) + 1, 0);
+    SegUnit U = {(int32_t)E0, (int32_t)(E0 + el), (int32_t)H.out.size(), (int32_t)(j - i), 0, 0, 0, flags};
+    int32_t lo = INT32_MAX, hi = -1;
+    int64_t at = E0;
+    for (int64_t q = i; q < j; q++) {
+      int64_t b, e;
+      int32_t o;
+      seg(q, b, e, o);
+      window(b, e, lo, hi);
+      H.start[(size_t)(at >> 3)] |= (uint8_t)(1u << (at & 7));
+      H.out.push_back(o);
+      H.col.insert(H.col.end(), scol + b, scol + e);
+      H.val.insert(H.val.end(), sval + b, sval + e);
+      at += e - b;
+    }
+    U.o0 = H.out[U.s0];
+    U.cmin = hi >= 0 ? lo : 0;
+    U.cwin = hi >= 0 ? hi - lo + 1 : 0;
+    if (staged) {
+      U.flags |= SEG_STAGE;
+      H.stage = true;
+    }
+    H.units.push_back(U);
+    H.pos = E0 + el;
+    i = j;
+  }
+}
+
+// The long rows' segments: slice (b, r) = the nonzeros of long row r (columns
+// sorted) that fall in column block b, ordered by block, then row, empty
+// slices left out; slice k sums into partial[k], and lsp/lseg list each
+// row's slices for the reduction.  A unit holds slices of one column block
+// only, so it stages one kColBlock-wide window of x in LDS.
+static void seg_long(SegHost &H, const std::vector<int32_t> &lrp, const std::vector<int32_t> &lcol,
+                     const std::vector<double> &lval, int nlong, int nblocks, int64_t target,
+                     std::vector<int32_t> &lsp, std::vector<int32_t> &lseg) {
+  static_assert(kColBlock <= kSegStage, "a column block must fit the LDS stage");
   std::vector<int64_t> cut((size_t)nlong * (nblocks + 1));  // cut[r*(nblocks+1)+b]: first element of r in block b
   for (int r = 0; r < nlong; r++) {
     int64_t j = lrp[r];
@@ -198,21 +306,92 @@ static void long_block_csr(const std::vector<int32_t> &lrp, const std::vector<in
       cut[(size_t)r * (nblocks + 1) + b] = b == nblocks ? lrp[r + 1] : j;
     }
   }
-  brp.assign((size_t)nblocks * nlong + 1, 0);
-  for (int b = 0; b < nblocks; b++)
-    for (int r = 0; r < nlong; r++) {
-      const size_t k = (size_t)b * nlong + r;
-      brp[k + 1] = brp[k] + (cut[(size_t)r * (nblocks + 1) + b + 1] - cut[(size_t)r * (nblocks + 1) + b]);
-    }
-  bcol.resize(lcol.size());
-  bval.resize(lval.size());
+  std::vector<int64_t> sb, se;
+  std::vector<int32_t> srow, sblk;  // long row, column block
   for (int b = 0; b < nblocks; b++)
     for (int r = 0; r < nlong; r++) {
       const int64_t s0 = cut[(size_t)r * (nblocks + 1) + b], s1 = cut[(size_t)r * (nblocks + 1) + b + 1];
-      const int64_t d0 = brp[(size_t)b * nlong + r];
-      std::copy(lcol.begin() + s0, lcol.begin() + s1, bcol.begin() + d0);
-      std::copy(lval.begin() + s0, lval.begin() + s1, bval.begin() + d0);
+      if (s1 <= s0) continue;
+      sb.push_back(s0);
+      se.push_back(s1);
+      srow.push_back(r);
+      sblk.push_back(b);
     }
+  lsp.assign(nlong + 1, 0);
+  for (int32_t r : srow) lsp[r + 1]++;
+  for (int r = 0; r < nlong; r++) lsp[r + 1] += lsp[r];
+  lseg.assign(srow.size(), 0);
+  std::vector<int32_t> fill(lsp.begin(), lsp.end() - 1);
+  for (size_t k = 0; k < srow.size(); k++) lseg[fill[srow[k]]++] = (int32_t)k;  // column order per row
+  // a block with more than `target` elements is cut into that many nearly
+  // equal units (10% slack for whole slices), each staging the block's x
+  const int64_t ns = (int64_t)sb.size();
+  for (int64_t i = 0; i < ns;) {
+    int64_t j = i, tot = 0;
+    while (j < ns && sblk[j] == sblk[i]) tot += se[j] - sb[j], j++;
+    const int64_t parts = (tot + target - 1) / target;
+    const int64_t tb = parts > 1 ? (tot * 11 + parts * 10 - 1) / (parts * 10) : target;
+    seg_append(
+        H, j - i,
+        [&](int64_t q, int64_t &b, int64_t &e, int32_t &o) {
+          b = sb[i + q];
+          e = se[i + q];
+          o = (int32_t)(i + q);
+        },
+        [](int64_t) { return false; }, lcol.data(), lval.data(), tb, true, SEG_PARTIAL);
+    i = j;
+  }
+}
+
+// Upload the segmented engine's arrays (padded so a group of 8 past the last
+// nonzero, and the start byte after it, are readable).
+static int seg_upload(spx_plan *P, SegHost &H) {
+  const int64_t total = ((H.pos + 7) & ~(int64_t)7) + 8;
+  H.col.resize(total, 0);
+  H.val.resize(total, 0.0);
+  H.start.resize((size_t)(total >> 3), 0);
+  // the kernel's preconditions, checked once on the host
+  const int64_t nslots = P->nlseg;
+  for (const SegUnit &U : H.units) {
+    bool ok = U.e0 % 8 == 0 && U.e0 < U.e1 && U.e1 <= H.pos && U.ns >= 1 && U.s0 >= 0 && U.o0 >= 0 &&
+              (int64_t)U.o0 + U.ns <= ((U.flags & SEG_PARTIAL) ? nslots : (int64_t)P->m) &&
+              (int64_t)U.s0 + U.ns <= (int64_t)H.out.size() && U.cmin >= 0 && (int64_t)U.cmin + U.cwin <= P->n &&
+              (!(U.flags & SEG_STAGE) || U.cwin <= kSegStage) && (H.start[U.e0 >> 3] & 1);
+    int64_t starts = 0;
+    for (int32_t k = U.e0; ok && k < U.e1; k++) {
+      starts += (H.start[k >> 3] >> (k & 7)) & 1;
+      ok = H.col[k] >= U.cmin && H.col[k] < U.cmin + U.cwin;
+    }
+    ok = ok && starts == U.ns;
+    for (int s = 0; ok && s < U.ns; s++) ok = H.out[U.s0 + s] == U.o0 + s;  // consecutive outputs
+    if (!ok) return fail(SPX_EDEV, "segmented engine: inconsistent unit table");
+  }
+  P->nsunits = (int64_t)H.units.size();
+  P->sstage = H.stage;
+  // XCD ranges of about equal work: nonzeros, staged x entries, and a fixed
+  // cost per unit
+  std::vector<double> wsum(H.units.size() + 1, 0.0);
+  for (size_t u = 0; u < H.units.size(); u++) {
+    const SegUnit &U = H.units[u];
+    wsum[u + 1] = wsum[u] + (U.e1 - U.e0) + ((U.flags & SEG_STAGE) ? 0.5 * U.cwin : 0.0) + 256.0;
+  }
+  std::vector<int32_t> xr(kXcd + 1, 0);
+  for (int x = 1; x < kXcd; x++)
+    xr[x] = (int32_t)(std::lower_bound(wsum.begin(), wsum.end(), wsum.back() * x / kXcd) - wsum.begin());
+  xr[kXcd] = (int32_t)H.units.size();
+  for (int x = 1; x <= kXcd; x++) xr[x] = std::max(xr[x], xr[x - 1]);
+  int rc;
+  if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
+  if ((rc = P->upload(&P->d_sxr, xr.data(), kXcd + 1))) return rc;
+  if ((rc = P->upload(&P->d_sstart, H.start.data(), (int64_t)H.start.size()))) return rc;
+  if ((rc = P->upload(&P->d_scol, H.col.data(), total))) return rc;
+  if (P->precision == SPX_PREC_F32) {
+    std::vector<float> v32(H.val.begin(), H.val.end());
+    if ((rc = P->upload(&P->d_sval32, v32.data(), total))) return rc;
+  } else {
+    if ((rc = P->upload(&P->d_sval, H.val.data(), total))) return rc;
+  }
+  return SPX_OK;
 }
 
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
@@ -220,7 +399,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int64_t nnz = d->nnz;
   if (m < 0 || n < 0 || nnz < 0) return fail(SPX_EARG, "negative size");
   if (nnz >= ((int64_t)1 << 31) - 1) return fail(SPX_ERANGE, "nnz must be below 2^31 per plan");
-  if (d->kernel < 1 || d->kernel > 3) return fail(SPX_EARG, "kernel must be 1, 2 or 3");
+  if (d->kernel < 1 || d->kernel > 4) return fail(SPX_EARG, "kernel must be 1, 2, 3 or 4");
   if (d->mode < 0 || d->mode > 2) return fail(SPX_EARG, "bad mode");
   if (d->precision != SPX_PREC_F64 && d->precision != SPX_PREC_F32) return fail(SPX_EARG, "bad precision");
   if (m > 0 && !d->rowptr64 && !d->rowptr32) return fail(SPX_EARG, "missing row pointer");
@@ -283,15 +462,16 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   bool src_dev = on_dev;
   std::vector<int32_t> hcol, rcol, lcol, lrp;
   std::vector<double> hval, rval, lval;
+  if (on_dev && (P->nlong > 0 || d->kernel == SPX_KERNEL_SEG)) {
+    hcol.resize(nnz);
+    hval.resize(nnz);
+    HIP_TRY(hipMemcpy(hcol.data(), d->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hval.data(), d->val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    src_col = hcol.data();
+    src_val = hval.data();
+    src_dev = false;
+  }
   if (P->nlong > 0) {
-    if (on_dev) {
-      hcol.resize(nnz);
-      hval.resize(nnz);
-      HIP_TRY(hipMemcpy(hcol.data(), d->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
-      HIP_TRY(hipMemcpy(hval.data(), d->val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
-      src_col = hcol.data();
-      src_val = hval.data();
-    }
     std::vector<char> is_long(m, 0);
     for (int r : long_rows) is_long[r] = 1;
     int64_t ln = 0;
@@ -339,6 +519,58 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   }
   const int64_t nnzr = rp[m];
   P->nnz_reg = nnzr;
+
+  // ---- long rows: (column block, row) slices for the segmented engine ----
+  SegHost H;
+  const int64_t seg_target = (int64_t)kBlock * 8 * (d->kernel == SPX_KERNEL_SEG && d->items > 0 ? d->items : 2);
+  if (d->kernel == SPX_KERNEL_SEG && (d->items < 0 || d->items > 32))
+    return fail(SPX_EARG, "items (chunks per unit) must be in 1..32");
+  if (P->nlong > 0) {
+    P->nblocks = (int)(((int64_t)n + kColBlock - 1) / kColBlock);
+    if ((int64_t)P->nblocks * P->nlong >= INT32_MAX) return fail(SPX_ERANGE, "too many long-row blocks");
+    if (P->long_nnz + 8 * (int64_t)P->nblocks * P->nlong + 64 >= INT32_MAX)
+      return fail(SPX_ERANGE, "padded long-row nonzeros must stay below 2^31 per plan");
+    std::vector<int32_t> lsp, lseg;
+    seg_long(H, lrp, lcol, lval, P->nlong, P->nblocks, std::max<int64_t>(seg_target, 8 * kBlock * 8), lsp, lseg);
+    P->nslong = (int64_t)H.units.size();
+    P->nlseg = (int64_t)lseg.size();
+    int rc;
+    if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
+    if ((rc = P->upload(&P->d_lsp, lsp.data(), P->nlong + 1))) return rc;
+    if ((rc = P->upload(&P->d_lseg, lseg.data(), P->nlseg))) return rc;
+    if ((rc = P->alloc(&P->d_partial, P->nlseg))) return rc;
+  }
+
+  // ---- kernel 4: the regular rows join the long slices in one launch ----
+  if (d->kernel == SPX_KERNEL_SEG) {
+    P->items = (int)(seg_target / (kBlock * 8));
+    std::vector<int32_t> rows, empty;
+    for (int i = 0; i < m; i++) (rp[i + 1] > rp[i] ? rows : empty).push_back(i);
+    if (P->long_nnz + nnzr + 8 * ((int64_t)rows.size() + (int64_t)P->nblocks * P->nlong) + 64 >= INT32_MAX)
+      return fail(SPX_ERANGE, "padded nonzeros must stay below 2^31 per plan");
+    seg_append(
+        H, (int64_t)rows.size(),
+        [&](int64_t i, int64_t &b, int64_t &e, int32_t &o) {
+          o = rows[i];
+          b = rp[o];
+          e = rp[o + 1];
+        },
+        [&](int64_t i) { return rows[i] != rows[i - 1] + 1; }, src_col, src_val, seg_target, true, 0);
+    int rc;
+    if ((rc = seg_upload(P, H))) return rc;
+    P->nempty = (int64_t)empty.size();
+    if ((rc = P->upload(&P->d_empty, empty.data(), P->nempty))) return rc;
+    if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    return SPX_OK;
+  }
+  if (P->nlong > 0) {
+    int rc;
+    if ((rc = seg_upload(P, H))) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&P->ldone, hipEventDisableTiming));
+  }
   const hipMemcpyKind h2d = src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
 
   // ---- partitions of the regular part ----
@@ -501,33 +733,6 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   }
   if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
 
-  // ---- long-row payload: block-major CSR swept by k_long ----
-  if (P->nlong > 0) {
-    P->nblocks = (int)(((int64_t)n + kColBlock - 1) / kColBlock);
-    if ((int64_t)P->nblocks * P->nlong >= INT32_MAX) return fail(SPX_ERANGE, "too many long-row blocks");
-    std::vector<int64_t> brp;
-    std::vector<int32_t> bcol;
-    std::vector<double> bval;
-    long_block_csr(lrp, lcol, lval, P->nlong, P->nblocks, brp, bcol, bval);
-    // padded so k_long can read whole groups of 8 past the last nonzero
-    bcol.resize(P->long_nnz + kLongPad, 0);
-    bval.resize(P->long_nnz + kLongPad, 0.0);
-    std::vector<int32_t> brp32(brp.begin(), brp.end());
-    if ((rc = P->upload(&P->d_brp, brp32.data(), (int64_t)brp32.size()))) return rc;
-    if ((rc = P->upload(&P->d_bcol, bcol.data(), P->long_nnz + kLongPad))) return rc;
-    if (P->precision == SPX_PREC_F32) {
-      std::vector<float> b32(bval.begin(), bval.end());
-      if ((rc = P->upload(&P->d_bval32, b32.data(), P->long_nnz + kLongPad))) return rc;
-    } else {
-      if ((rc = P->upload(&P->d_bval, bval.data(), P->long_nnz + kLongPad))) return rc;
-    }
-    if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
-    if ((rc = P->alloc(&P->d_partial, (int64_t)P->nblocks * P->nlong))) return rc;
-    HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&P->ldone, hipEventDisableTiming));
-  }
-
   // ---- launch layout: partitions -> streams ----
   int nstreams = 1;
   if (P->mode == SPX_MODE_QUEUE) {
@@ -687,6 +892,17 @@ static KArgs<V, X> make_args(spx_plan *P, const V *val, const X *x, double alpha
   return a;
 }
 
+static hipError_t launch_seg_all(spx_plan *P, const double *x, double alpha, double beta, double *y, hipStream_t st) {
+  if (P->precision == SPX_PREC_F32) {
+    SegArgs<float, XPair> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol, P->d_sval32, P->d_xhl,
+                               alpha, beta, y, P->d_partial};
+    return launch_seg_f32(a, P->sstage, st);
+  }
+  SegArgs<double, double> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol, P->d_sval, x,
+                               alpha, beta, y, P->d_partial};
+  return launch_seg_f64(a, P->sstage, st);
+}
+
 // Enqueue one SpMV.  t0/t1 (optional) bracket the SpMV kernels proper (the
 // regular partitions and the long-row sweep, which overlap); x splitting,
 // the fix-up and the long-row reduction fall outside.  `dry` runs only the
@@ -698,17 +914,21 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
   if (!y || (P->n > 0 && !x)) return fail(SPX_EARG, "null x or y");
   const bool f32 = P->precision == SPX_PREC_F32;
   if (f32) HIP_TRY(launch_split_x(x, P->d_xhl, P->n, st));
+  if (P->kernel == SPX_KERNEL_SEG) {
+    // empty rows first (long rows included: the reduction adds to beta*y0)
+    HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
+    if (t0) HIP_TRY(hipEventRecord(t0, st));
+    HIP_TRY(launch_seg_all(P, x, alpha, beta, y, st));
+    if (t1) HIP_TRY(hipEventRecord(t1, st));
+    if (P->nlong > 0) HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->d_lsp, P->d_lseg, P->d_lrows, alpha, y, st));
+    return SPX_OK;
+  }
   if (t0) HIP_TRY(hipEventRecord(t0, st));
   const bool longpath = P->nlong > 0 && !dry;
   if (longpath) {
     HIP_TRY(hipEventRecord(P->lfork, st));
     HIP_TRY(hipStreamWaitEvent(P->lstream, P->lfork, 0));
-    if (f32)
-      HIP_TRY(launch_long_f32(P->d_brp, P->nlong, P->nblocks, P->d_bcol, P->d_bval32, P->d_xhl, P->n, P->d_partial,
-                              P->lstream));
-    else
-      HIP_TRY(launch_long_f64(P->d_brp, P->nlong, P->nblocks, P->d_bcol, P->d_bval, x, P->n, P->d_partial,
-                              P->lstream));
+    HIP_TRY(launch_seg_all(P, x, alpha, beta, y, P->lstream));
     HIP_TRY(hipEventRecord(P->ldone, P->lstream));
   }
   if (P->kernel == SPX_KERNEL_TILE) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
@@ -725,7 +945,7 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
   if (t1) HIP_TRY(hipEventRecord(t1, st));
   if (dry) return SPX_OK;
   HIP_TRY(launch_fixup(P->d_runs, P->nruns, P->d_crow, P->d_cval, alpha, beta, y, st));
-  if (longpath) HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->nblocks, P->d_lrows, alpha, y, st));
+  if (longpath) HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->d_lsp, P->d_lseg, P->d_lrows, alpha, y, st));
   return SPX_OK;
 }
 
@@ -749,7 +969,8 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
 int spx_plan_info(spx_plan_t p, int64_t *out, int cap) {
   if (!p || !out) return 0;
   int64_t v[] = {p->m,      p->n,        p->nnz,       p->kernel, p->mode,   (int64_t)p->parts.size(),
-                 p->sigma,  p->lanes,    p->items,     p->nunits, p->ncarry, p->ntiles,
+                 p->sigma,  p->lanes,    p->items,
+                 p->kernel == SPX_KERNEL_SEG ? p->nsunits : p->nunits, p->ncarry, p->ntiles,
                  p->bytes,  p->precision, p->queues,   p->nlong,  p->long_nnz, p->nblocks,
                  p->nruns,  p->long_th};
   int c = std::min<int>(cap, (int)(sizeof(v) / sizeof(v[0])));

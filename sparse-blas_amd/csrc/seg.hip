@@ -1,0 +1,256 @@
+// seg.hip -- the segmented engine: row-aligned units, 16-byte loads, LDS x.
+//
+// A "segment" is a run of nonzeros that sums into one output: a nonempty row
+// (plain SpMV) or the slice of a long row inside one column block.  Segments
+// are grouped into units at segment boundaries, so no segment is ever split
+// between workgroups and no carries leave a unit.  Each unit's nonzeros are
+// copied to an 8-aligned region (padding <= 7 entries), so a thread owns 8
+// aligned nonzeros per chunk and fetches them with 16-byte loads, plus one
+// byte of segment-start bits.  A unit's segments write consecutive outputs,
+// so a sum's destination follows from its index without a further load.  A
+// unit flagged SEG_STAGE first stages its
+// column window of x (at most kSegStage entries) in LDS, and every gather
+// hits LDS; a unit flagged SEG_PARTIAL stores its segment sums into a
+// partial array instead of updating y.
+//
+// Per chunk of kBlock*8 nonzeros, each thread splits its 8 products at its
+// start bits into pieces.  Pieces bounded on both sides inside the thread are
+// whole segments and are written out at once.  The piece that continues from
+// earlier threads and the piece that runs on past the thread are joined by a
+// block-wide segmented scan (wave shuffles, then one exchange of the four
+// wave totals through LDS), which also counts start bits to give every piece
+// its segment index.  The sum still open at the end of a chunk carries into
+// the next chunk in registers.  One barrier per chunk: the wave totals are
+// double-buffered.  Every addition happens in a fixed order, so the result
+// is bitwise reproducible.
+#include "spx_device.h"
+
+namespace spx {
+
+template <typename V>
+struct Vec8;
+template <>
+struct Vec8<double> {
+  __device__ static inline void load(const double *p, double *o) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const double2 u = reinterpret_cast<const double2 *>(p)[i];
+      o[2 * i] = u.x;
+      o[2 * i + 1] = u.y;
+    }
+  }
+};
+template <>
+struct Vec8<float> {
+  __device__ static inline void load(const float *p, float *o) {
+    const float4 u = reinterpret_cast<const float4 *>(p)[0], w = reinterpret_cast<const float4 *>(p)[1];
+    o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
+  }
+};
+__device__ inline void load8i(const int32_t *p, int32_t *o) {
+  const int4 u = reinterpret_cast<const int4 *>(p)[0], w = reinterpret_cast<const int4 *>(p)[1];
+  o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
+}
+
+// Unit for this workgroup's iteration `iter`, or -1 when done.  XCD x owns
+// units [xr[x], xr[x+1]), ranges the host balanced by work, and its
+// workgroups walk them round-robin in order, so neighbouring units (which
+// share x entries) run at the same time in that XCD's L2.  No atomics: a
+// shared counter serialises at memory once hundreds of workgroups pop it.
+__device__ inline int seg_unit(const int32_t *xr, int iter) {
+  const int x = blockIdx.x % kXcd;
+  const int per = gridDim.x / kXcd;
+  const int u = xr[x] + (int)(blockIdx.x / kXcd) + iter * per;
+  return u < xr[x + 1] ? u : -1;
+}
+
+template <typename V, typename X>
+__device__ inline void seg_emit(const SegArgs<V, X> &a, bool part, int o, double s) {
+  if (part)
+    a.partial[o] = s;
+  else
+    a.y[o] = a.beta == 0.0 ? a.alpha * s : a.alpha * s + a.beta * a.y[o];
+}
+
+template <typename V, typename X, bool STAGE>
+__global__ __launch_bounds__(kBlock) void k_seg(SegArgs<V, X> a) {
+  constexpr int LE = 8;
+  constexpr int CH = kBlock * LE;
+  constexpr int NW = kBlock / kWave;
+  __shared__ X s_x[STAGE ? kSegStage : 1];
+  __shared__ double s_wv[2][NW];  // per wave: scan value, closed flag, start-bit count
+  __shared__ int s_ws[2][NW];
+  __shared__ int s_wc[2][NW];
+  const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+  int par = 0;  // double-buffer slot of the wave totals, alternating per chunk
+  for (int iter = 0;; iter++) {
+    const int u = seg_unit(a.xr, iter);
+    if (u < 0) break;
+    const SegUnit U = a.units[u];
+    const bool stg = STAGE && (U.flags & SEG_STAGE);
+    const bool part = (U.flags & SEG_PARTIAL) != 0;
+    if (stg) {
+      __syncthreads();  // the previous unit's gathers from s_x are done
+      for (int k0 = 0; k0 < U.cwin; k0 += 4 * kBlock) {
+        X xv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int k = k0 + tid + i * kBlock;
+          if (k < U.cwin) xv[i] = a.x[U.cmin + k];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int k = k0 + tid + i * kBlock;
+          if (k < U.cwin) s_x[k] = xv[i];
+        }
+      }
+      __syncthreads();
+    }
+    double carry = 0.0;  // sum still open at the end of the previous chunk
+    int base = U.o0;     // outputs of the segments begun before the current chunk
+    for (int c = U.e0; c < U.e1; c += CH, par ^= 1) {
+      const int ea = c + tid * LE;
+      const int nv = U.e1 - ea <= 0 ? 0 : (U.e1 - ea > LE ? LE : U.e1 - ea);
+      double p[LE];
+      uint32_t fb = 0;
+      int tclosed = 1;  // the thread's last piece ends with it
+      {
+        int32_t ci[LE];
+        V vv[LE];
+        if (nv > 0) {  // the arrays are padded past every unit: a group of 8 is readable
+          load8i(a.col + ea, ci);
+          Vec8<V>::load(a.val + ea, vv);
+          fb = a.start[ea >> 3];
+          if (nv < LE) fb &= (1u << nv) - 1;
+          if (ea + nv < U.e1) tclosed = a.start[(ea + LE) >> 3] & 1;
+        } else {
+#pragma unroll
+          for (int i = 0; i < LE; i++) {
+            ci[i] = 0;
+            vv[i] = (V)0;
+          }
+        }
+        if (stg) {
+#pragma unroll
+          for (int i = 0; i < LE; i++) {
+            const int cc = i < nv ? ci[i] - U.cmin : 0;  // in [0, cwin) for real nonzeros
+            p[i] = i < nv ? prod(vv[i], s_x, cc) : 0.0;
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < LE; i++) {
+            const int cc = i < nv ? ci[i] : 0;
+            p[i] = i < nv ? prod(vv[i], a.x, cc) : 0.0;
+          }
+        }
+      }
+      // piece 0 = products before the first inner start bit; the last piece
+      // starts at the last start bit (piece 0 when there is none)
+      const uint32_t inner = fb & ~1u;
+      const int f1 = inner ? __builtin_ctz(inner) : nv;
+      const int fl = inner ? 31 - __builtin_clz(inner) : 0;
+      double first_val = 0.0, last_val = 0.0;
+#pragma unroll
+      for (int i = 0; i < LE; i++) {
+        if (i < f1) first_val += p[i];
+        if (i >= fl && i < nv) last_val += p[i];
+      }
+      const int has_close = nv > 0 && (inner != 0 || tclosed);
+      const double run = nv > 0 && !tclosed ? last_val : 0.0;
+      const int cnt = __builtin_popcount(fb);
+      // segmented inclusive scan c(t) = run(t) + (has_close(t) ? 0 : c(t-1)),
+      // and an inclusive scan of the start-bit counts
+      double v = run;
+      int sg = has_close, cn = cnt;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const double vu = __shfl_up(v, off, kWave);
+        const int su = __shfl_up(sg, off, kWave);
+        const int cu = __shfl_up(cn, off, kWave);
+        if (lane >= off) {
+          cn += cu;
+          if (!sg) {
+            v += vu;
+            sg |= su;
+          }
+        }
+      }
+      if (lane == kWave - 1) {
+        s_wv[par][wid] = v;
+        s_ws[par][wid] = sg;
+        s_wc[par][wid] = cn;
+      }
+      __syncthreads();
+      double prev = carry, tot = carry;
+      int cb = base, totc = base;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const double wv = s_wv[par][w];
+        const int ws = s_ws[par][w], wc = s_wc[par][w];
+        if (w < wid) {
+          prev = wv + (ws ? 0.0 : prev);
+          cb += wc;
+        }
+        tot = wv + (ws ? 0.0 : tot);
+        totc += wc;
+      }
+      const double c_incl = v + (sg ? 0.0 : prev);
+      double carry_in = __shfl_up(c_incl, 1, kWave);
+      if (lane == 0) carry_in = prev;
+      const int excl = cb + cn - cnt;  // output of the first segment begun by this thread
+      if (nv > 0) {
+        // piece 0 closes the segment open before this thread (or one that
+        // begins at its first product)
+        if (has_close) seg_emit(a, part, excl + (int)(fb & 1) - 1, first_val + carry_in);
+        // pieces beginning at inner start bits
+        double acc = 0.0;
+        int open = 0, k = excl + (int)(fb & 1) - 1;
+#pragma unroll
+        for (int i = 1; i < LE; i++) {
+          if (i < nv) {
+            if ((inner >> i) & 1) {
+              if (open) seg_emit(a, part, k, acc);
+              k++;
+              acc = 0.0;
+              open = 1;
+            }
+            if (open) acc += p[i];
+          }
+        }
+        if (open && tclosed) seg_emit(a, part, k, acc);
+      }
+      carry = tot;
+      base = totc;
+    }
+  }
+}
+
+template <typename K>
+static int seg_cap(K kern) {
+  int nb = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  const int cap = nb * cus / kXcd * kXcd;
+  return cap < kXcd ? kXcd : cap;
+}
+
+template <typename V, typename X, bool STAGE>
+static hipError_t seg_launch(const SegArgs<V, X> &a, hipStream_t st) {
+  if (a.nunits <= 0) return hipSuccess;
+  static const int cap = seg_cap(k_seg<V, X, STAGE>);
+  int g = direct_grid(a.nunits);
+  if (g > cap) g = cap;
+  hipLaunchKernelGGL((k_seg<V, X, STAGE>), dim3(g), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st) {
+  return stage ? seg_launch<double, double, true>(a, st) : seg_launch<double, double, false>(a, st);
+}
+hipError_t launch_seg_f32(const SegArgs<float, XPair> &a, bool stage, hipStream_t st) {
+  return stage ? seg_launch<float, XPair, true>(a, st) : seg_launch<float, XPair, false>(a, st);
+}
+
+}  // namespace spx

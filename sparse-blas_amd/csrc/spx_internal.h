@@ -13,8 +13,7 @@ constexpr int kBlock = 256;   // threads per workgroup (4 wave64s)
 constexpr int kWave = 64;     // CDNA wavefront width
 constexpr int kOmega = 64;    // lanes per CSR5-style tile
 constexpr int kXcd = 8;       // accelerator complex dies (one L2 each)
-constexpr int kColBlock = 2048;   // long-row path: x entries staged in LDS per column block
-constexpr int kLongPad = 16;      // long-row path: padding of the block-major arrays
+constexpr int kColBlock = 2048;   // long-row path: column block width (x staged in LDS)
 constexpr int kMaxLong = 1024;    // long-row path: most long rows per plan
 
 // Workgroups for a direct launch over `count` units: a multiple of kXcd so
@@ -63,6 +62,34 @@ struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
   float hi, lo;
 };
 
+// ---- segmented engine (seg.hip) ----------------------------------------------
+constexpr int kSegStage = 3968;  // most x entries a unit stages in LDS (5 workgroups fit a CU)
+
+// One workgroup unit of the segmented engine: padded elements [e0, e1)
+// (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order, with
+// columns in [cmin, cmin + cwin).  Segment s0 + j writes output o0 + j: y row
+// o0 + j, or partial slot o0 + j in SEG_PARTIAL units.  Bit (e & 7) of
+// start[e >> 3] is set when element e begins a segment.
+struct SegUnit {
+  int32_t e0, e1, s0, ns, o0, cmin, cwin, flags;
+};
+
+enum { SEG_STAGE = 1, SEG_PARTIAL = 2 };  // SegUnit.flags
+
+template <typename V, typename X>
+struct SegArgs {
+  const SegUnit *units;
+  int64_t nunits;
+  const int32_t *xr;     // XCD x owns units [xr[x], xr[x + 1]) (kXcd + 1 entries)
+  const uint8_t *start;  // segment-start bits, one per padded element
+  const int32_t *col;
+  const V *val;
+  const X *x;
+  double alpha, beta;
+  double *y;
+  double *partial;
+};
+
 template <typename V, typename X>
 struct KArgs {
   const Part *parts;
@@ -107,14 +134,10 @@ hipError_t launch_empty_rows(const int32_t *rows, int64_t nrows, double beta,
 hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow,
                         const double *cval, double alpha, double beta, double *y,
                         hipStream_t st);
-hipError_t launch_long_f64(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol,
-                           const double *bval, const double *x, int n, double *partial,
-                           hipStream_t st);
-hipError_t launch_long_f32(const int32_t *brp, int nlong, int nblocks, const int32_t *bcol,
-                           const float *bval, const XPair *x, int n, double *partial,
-                           hipStream_t st);
-hipError_t launch_long_reduce(const double *partial, int nlong, int nblocks,
-                              const int32_t *rows, double alpha, double *y, hipStream_t st);
+// y[rows[r]] += alpha * sum of partial[lseg[j]], j in [lsp[r], lsp[r+1]).
+hipError_t launch_long_reduce(const double *partial, int nlong, const int32_t *lsp,
+                              const int32_t *lseg, const int32_t *rows, double alpha, double *y,
+                              hipStream_t st);
 hipError_t launch_reset(int *q, int n, hipStream_t st);
 hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag,
                              hipStream_t st);
@@ -126,5 +149,9 @@ hipError_t launch_merge_f64(const KArgs<double, double> &a, int items, int grid,
 hipError_t launch_merge_f32(const KArgs<float, XPair> &a, int items, int grid, hipStream_t st);
 hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st);
 hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st);
+
+// Segmented engine: `stage` = some unit stages x in LDS (SEG_STAGE).
+hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st);
+hipError_t launch_seg_f32(const SegArgs<float, XPair> &a, bool stage, hipStream_t st);
 
 }  // namespace spx

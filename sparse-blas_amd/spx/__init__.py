@@ -17,7 +17,7 @@ GEN_PATH = os.path.join(_PKG, "libspxgen.so")
 
 OK, EARG, EMEM, EDEV, EGUARD, ERANGE = 0, -1, -2, -3, -4, -5
 MODE_ROWS, MODE_NNZ, MODE_QUEUE = 0, 1, 2
-KERNEL_VECTOR, KERNEL_MERGE, KERNEL_TILE = 1, 2, 3
+KERNEL_VECTOR, KERNEL_MERGE, KERNEL_TILE, KERNEL_SEG = 1, 2, 3, 4
 PREC_F64, PREC_F32 = 0, 1
 FLAG_DEVICE_PTRS = 1
 

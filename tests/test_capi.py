@@ -123,7 +123,8 @@ def test_plan_create_validates_before_touching_the_device():
     v = np.array([1.0, 2.0, 3.0])
     h = C.c_void_p()
     base = dict(m=2, n=2, nnz=3, rowptr64=rp.ctypes.data, col=c.ctypes.data, val=v.ctypes.data)
-    assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, kernel=4))) == spx.EARG
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, kernel=5))) == spx.EARG
+    assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, kernel=4, items=33))) == spx.EARG
     assert L.spx_plan_create(C.byref(h), C.byref(desc(**base, precision=7))) == spx.EARG
     assert L.spx_plan_create(C.byref(h), C.byref(desc(**dict(base, rowptr64=bad_rp.ctypes.data)))) == spx.EARG
     assert L.spx_plan_create(C.byref(h), C.byref(desc(**dict(base, nnz=4)))) == spx.EARG

@@ -28,8 +28,9 @@ def check_members(mats, y, x, x_off, y_off, alpha, beta, y0, tol=1e-13):
 
 @pytest.mark.parametrize("kc", [dict(kernel=1), dict(kernel=2), dict(kernel=3),
                                 dict(kernel=2, mode=spx.MODE_NNZ, parts=3),
-                                dict(kernel=2, precision=spx.PREC_F32)],
-                         ids=["k1", "k2", "k3", "k2-parts3", "k2-f32"])
+                                dict(kernel=2, precision=spx.PREC_F32), dict(kernel=4),
+                                dict(kernel=4, precision=spx.PREC_F32)],
+                         ids=["k1", "k2", "k3", "k2-parts3", "k2-f32", "k4", "k4-f32"])
 def test_batch_matches_oracle_per_matrix(kc):
     mats = members()
     P = spx.BatchPlan(mats, nb=20000, **kc)
@@ -42,8 +43,9 @@ def test_batch_matches_oracle_per_matrix(kc):
 
 @pytest.mark.parametrize("kc", [dict(kernel=2, mode=spx.MODE_QUEUE, nb=20000),
                                 dict(kernel=3, mode=spx.MODE_QUEUE, nb=7000, queues=2),
-                                dict(kernel=2, mode=spx.MODE_NNZ, parts=4, long_row=100)],
-                         ids=["queue", "queue-q2", "parts4-long"])
+                                dict(kernel=2, mode=spx.MODE_NNZ, parts=4, long_row=100),
+                                dict(kernel=4, long_row=100)],
+                         ids=["queue", "queue-q2", "parts4-long", "k4-long"])
 def test_graph_replays_match_eager_bitwise(kc):
     mats = members(16, seed=3)
     P = spx.BatchPlan(mats, **kc)

@@ -44,6 +44,7 @@ KERNEL_CASES = [
     dict(kernel=1), dict(kernel=1, lanes=64), dict(kernel=1, lanes=2),
     dict(kernel=2), dict(kernel=2, items=2), dict(kernel=2, items=16),
     dict(kernel=3), dict(kernel=3, sigma=4), dict(kernel=3, sigma=32), dict(kernel=3, sigma=7),
+    dict(kernel=4), dict(kernel=4, items=1), dict(kernel=4, items=8),
 ]
 MODE_CASES = [
     dict(mode=spx.MODE_NNZ, parts=1), dict(mode=spx.MODE_NNZ, parts=3), dict(mode=spx.MODE_NNZ, parts=8),
@@ -71,7 +72,7 @@ def test_c1_all_kernels_and_modes(c1, kc, mc):
     check_fp64(A, x, y, 0.7, -0.3, y0)
 
 
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 @pytest.mark.parametrize("mc", MODE_CASES, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
 def test_small_integers_bit_exact(kernel, mc):
     A = W.small_int(3000, 2000, 12, seed=5)
@@ -110,6 +111,9 @@ def edge_matrices():
     lens = rng.integers(1, 5, 2000)
     lens[[5, 900, 1999]] = [9000, 15000, 12000]
     mk(lens, 20000, "long_rows_at_edges")
+    mk(np.full(3000, 1), 3000, "single_nonzero_rows")  # more rows than one unit holds
+    mk(np.full(1024, 8), 2048, "eight_per_row")  # rows on 8-aligned boundaries
+    mk([7, 2049, 1, 8, 4093, 2], 5000, "chunk_straddling_rows")
     return out
 
 
@@ -117,7 +121,7 @@ EDGE = edge_matrices()
 
 
 @pytest.mark.parametrize("name", list(EDGE))
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 @pytest.mark.parametrize("parts", [1, 2, 5])
 def test_edge_cases(name, kernel, parts):
     A = EDGE[name]
@@ -205,7 +209,7 @@ def test_merge_coordinates_on_the_merge_path():
 def test_reproducible_bits():
     A = W.zipf(8192, 1.2, 2000, seed=2)
     x = W.vector(A["n"], 3)
-    for kc in ({"kernel": 2}, {"kernel": 3}, {"kernel": 1, "parts": 4}):
+    for kc in ({"kernel": 2}, {"kernel": 3}, {"kernel": 1, "parts": 4}, {"kernel": 4}):
         y1, _ = run_plan(A, x, **kc)
         y2, _ = run_plan(A, x, **kc)
         assert np.array_equal(y1, y2)
@@ -277,7 +281,8 @@ def c2():
 
 
 @pytest.mark.parametrize("kc", [dict(kernel=2), dict(kernel=3), dict(kernel=2, parts=8),
-                                dict(kernel=2, mode=spx.MODE_QUEUE, nb=59_500_000 // 8, queues=8)])
+                                dict(kernel=2, mode=spx.MODE_QUEUE, nb=59_500_000 // 8, queues=8),
+                                dict(kernel=4), dict(kernel=4, items=8)])
 def test_c2_circuit_full_size(c2, kc):
     A, x, ref = c2
     print("c2 case", kc, flush=True)
@@ -295,7 +300,7 @@ def test_c2_circuit_full_size(c2, kc):
 
 
 @pytest.mark.parametrize("name", ["rail", "asic"])
-@pytest.mark.parametrize("kernel", [2, 3])
+@pytest.mark.parametrize("kernel", [2, 3, 4])
 def test_precision_form(name, kernel):
     A = W.rail() if name == "rail" else W.asic()
     x = W.vector(A["n"], 31)
@@ -309,10 +314,10 @@ def test_precision_form(name, kernel):
         assert rel <= 1e-13
 
 
-# ---- long rows (column-blocked path with x staged in LDS) ------------------------------
+# ---- long rows (column-blocked slices, x staged in LDS) --------------------------------
 
 @pytest.mark.parametrize("long_row", [16, 300, -1])
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 @pytest.mark.parametrize("mc", [dict(parts=1), dict(parts=4), dict(mode=spx.MODE_QUEUE, nb=3000, queues=2)],
                          ids=["p1", "p4", "queue"])
 def test_long_row_path(long_row, kernel, mc):
@@ -343,7 +348,7 @@ def test_long_rows_unsorted_columns_and_bit_exact_integers():
         val[rp[r]:rp[r + 1]] = val[idx]
     B = W.finish(rp, col, val, A["m"], A["n"])
     x = W.vector(B["n"], 9, W.VAL_SMALLINT)
-    for kernel in (1, 2, 3):
+    for kernel in (1, 2, 3, 4):
         y, P = run_plan(B, x, 1.0, 0.0, None, kernel=kernel, long_row=50)
         assert P.info()["nlong"] > 0
         assert np.array_equal(y, oracle.spmv(rp, col, val, x, 1.0, 0.0))
@@ -353,6 +358,7 @@ def test_long_row_path_precision_form():
     A = W.zipf(6000, 1.1, 3000, seed=22)
     x = W.vector(A["n"], 7)
     ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
-    y, P = run_plan(A, x, kernel=2, long_row=100, precision=spx.PREC_F32)
-    assert P.info()["nlong"] > 0
-    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-7
+    for kernel in (2, 4):
+        y, P = run_plan(A, x, kernel=kernel, long_row=100, precision=spx.PREC_F32)
+        assert P.info()["nlong"] > 0
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-7
