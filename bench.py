@@ -6,7 +6,7 @@ columns, the rest 1..12 nonzeros near the diagonal), FP64 N(0,1) values, x
 U[-1,1), alpha = 1, beta = 0.  A, x and y are resident in HBM before the timed
 region.  One step = one SpMV (spx_plan_execute) over the whole matrix.
 
-    python bench.py [--steps K] [--warmup W] [--kernel 1|2|3] [--parts P]
+    python bench.py [--steps K] [--warmup W] [--kernel 1|2|3|4] [--items I] [--parts P]
 
 Prints ONE JSON line: value = algorithmic GB/s of the whole step (wall clock,
 synchronize on both sides), roofline = the dominant SpMV kernel's achieved
@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -33,19 +34,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel, items, window):
+    """HBM bytes per launch of the dominant kernel for this configuration, as
+    recorded from FETCH_SIZE + WRITE_SIZE runs in profiles/*.md, else None."""
+    pat = re.compile(r"traffic_bytes_per_launch: (\d+) \(kernel=(\d+) items=(\d+) window=(\d+)\)")
+    for name in sorted(os.listdir(os.path.join(ROOT, "profiles"))):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            for mt in pat.finditer(f.read()):
+                if (int(mt[2]), int(mt[3]), int(mt[4])) == (kernel, items, window):
+                    return int(mt[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--kernel", type=int, default=2)
+    ap.add_argument("--kernel", type=int, default=4)
     ap.add_argument("--parts", type=int, default=1)
-    ap.add_argument("--items", type=int, default=0)
+    ap.add_argument("--items", type=int, default=4)
     ap.add_argument("--sigma", type=int, default=0)
     ap.add_argument("--window", type=int, default=1024,
                     help="half-width of the regular rows' column band (0 = uniform columns)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-reps", type=int, default=100)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -78,12 +93,12 @@ def main():
         plan.execute(xd, yd, alpha, beta)
     spx.synchronize()
 
-    # timed region: K back-to-back executes on one stream, events per execute
+    # timed region: K back-to-back executes on one stream with HIP events
+    # around each execute and around its SpMV kernel; the library reads the
+    # host clock from the first timed enqueue to the end of the last execute
     spx.synchronize()
-    t_start = time.perf_counter()
-    tot_ms, ker_ms = plan.time(xd, yd, alpha, beta, warmup=0, reps=a.steps)
+    tot_ms, ker_ms, wall_ms = plan.time(xd, yd, alpha, beta, warmup=0, reps=a.steps, wall=True)
     spx.synchronize()
-    wall_ms = (time.perf_counter() - t_start) * 1e3 / a.steps
     log(f"timed {a.steps} steps: wall {wall_ms:.4f} ms/step, events {tot_ms:.4f} ms/execute, "
         f"kernel {ker_ms:.4f} ms")
 
@@ -118,13 +133,13 @@ def main():
                          f"uniform columns; other rows 1..12 nnz within +-{a.window} of the diagonal), "
                          "FP64 values, y = A x (alpha=1, beta=0)"),
             "kernel": a.kernel, "parts": info["parts"], "sigma": info["sigma"], "items": info["items"],
-            "mode": "nnz-balanced (v1)",
+            "mode": "segmented units, one launch" if a.kernel == 4 else "nnz-balanced partitions (v1)",
         },
         "gflops": round(2 * nnz / (wall_ms * 1e-3) / 1e9, 2),
         "ref_getB_gbps": round(W.ref_getB(m, nnz) / (wall_ms * 1e-3) / 1e9, 2),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(a.kernel, info["items"], a.window),
             "bytes_per_launch": B, "kernel_ms": round(ker_ms, 5),
         },
     }

@@ -124,12 +124,15 @@ int spx_plan_info(spx_plan_t p, int64_t *out, int cap);
  * when dst != NULL. */
 int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap);
 
-/* Average device time (ms) of each of `reps` executions after `warmup`
- * untimed ones, measured with HIP events on the plan's launch stream.
+/* Average time (ms) of each of `reps` executions after `warmup` untimed
+ * ones, enqueued back-to-back on one stream with HIP events around each.
  * out_ms[0] = whole execute (events around each execute), out_ms[1] = the
- * dominant SpMV kernel alone (partition kernels, fixup excluded).
+ * dominant SpMV kernel alone (partition kernels, fixup and long-row
+ * reduction excluded), out_ms[2] = host wall clock from the first timed
+ * enqueue to the end of the last execute, per execute.  out_ms holds 3.
  * If flush_bytes > 0 a buffer of that size is overwritten before every
- * timed execute so L2/MALL start cold (the flush itself is not timed). */
+ * timed execute so L2/MALL start cold (outside the events, inside the wall
+ * clock). */
 int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta,
                   double *y, int warmup, int reps, int64_t flush_bytes,
                   double *out_ms);

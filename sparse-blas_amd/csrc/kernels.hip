@@ -296,18 +296,27 @@ __global__ void k_fixup(const Run *__restrict__ runs, int64_t nruns, const int32
 // The segmented engine sweeps the long rows by column blocks and leaves one
 // partial sum per (column block, row) slice; lseg[lsp[r] .. lsp[r+1]) lists
 // row r's slices in column order.  One workgroup per long row adds them in a
-// fixed tree order: y[row] += alpha * sum (the regular pass has already
-// written beta * y0 for the row, which is empty there).
+// fixed tree order, then y[row] = alpha * sum + beta * y[row] (`assign`), or
+// y[row] += alpha * sum when the regular pass has already written beta * y0
+// for the row (it is empty there).
 __global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong,
                                                         const int32_t *__restrict__ lsp,
                                                         const int32_t *__restrict__ lseg,
                                                         const int32_t *__restrict__ rows, double alpha,
-                                                        double *__restrict__ y) {
+                                                        double beta, int assign, double *__restrict__ y) {
+  constexpr int U = 4;  // independent loads in flight per thread
   __shared__ double s_w[kBlock / kWave];
   const int r = blockIdx.x, tid = threadIdx.x;
   if (r >= nlong) return;
+  const int j0 = lsp[r], j1 = lsp[r + 1];
   double s = 0.0;
-  for (int j = lsp[r] + tid; j < lsp[r + 1]; j += kBlock) s += partial[lseg[j]];
+  for (int j = j0 + tid; j < j1; j += U * kBlock) {
+    double v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) v[k] = j + k * kBlock < j1 ? partial[lseg[j + k * kBlock]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < U; k++) s += v[k];
+  }
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
   if (tid % kWave == 0) s_w[tid / kWave] = s;
@@ -315,7 +324,11 @@ __global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict
   if (tid == 0) {
     double t = 0.0;
     for (int w = 0; w < kBlock / kWave; w++) t += s_w[w];
-    y[rows[r]] += alpha * t;
+    const int row = rows[r];
+    if (!assign)
+      y[row] += alpha * t;
+    else
+      y[row] = beta == 0.0 ? alpha * t : alpha * t + beta * y[row];
   }
 }
 
@@ -720,9 +733,11 @@ hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow, con
 }
 
 hipError_t launch_long_reduce(const double *partial, int nlong, const int32_t *lsp, const int32_t *lseg,
-                              const int32_t *rows, double alpha, double *y, hipStream_t st) {
+                              const int32_t *rows, double alpha, double beta, bool assign, double *y,
+                              hipStream_t st) {
   if (nlong <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_long_reduce, dim3(nlong), dim3(kBlock), 0, st, partial, nlong, lsp, lseg, rows, alpha, y);
+  hipLaunchKernelGGL(k_long_reduce, dim3(nlong), dim3(kBlock), 0, st, partial, nlong, lsp, lseg, rows, alpha, beta,
+                     assign ? 1 : 0, y);
   return hipGetLastError();
 }
 

@@ -134,10 +134,11 @@ hipError_t launch_empty_rows(const int32_t *rows, int64_t nrows, double beta,
 hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow,
                         const double *cval, double alpha, double beta, double *y,
                         hipStream_t st);
-// y[rows[r]] += alpha * sum of partial[lseg[j]], j in [lsp[r], lsp[r+1]).
+// With S = sum of partial[lseg[j]], j in [lsp[r], lsp[r+1]): y[rows[r]] =
+// alpha*S + beta*y[rows[r]] (`assign`), else y[rows[r]] += alpha*S.
 hipError_t launch_long_reduce(const double *partial, int nlong, const int32_t *lsp,
-                              const int32_t *lseg, const int32_t *rows, double alpha, double *y,
-                              hipStream_t st);
+                              const int32_t *lseg, const int32_t *rows, double alpha, double beta,
+                              bool assign, double *y, hipStream_t st);
 hipError_t launch_reset(int *q, int n, hipStream_t st);
 hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag,
                              hipStream_t st);

@@ -235,10 +235,12 @@ class Plan:
                 raise SpxError(EDEV, "export copy failed")
         return out
 
-    def time(self, x_dev, y_dev, alpha=1.0, beta=0.0, warmup=5, reps=20, flush_bytes=0):
-        out = (C.c_double * 2)()
+    def time(self, x_dev, y_dev, alpha=1.0, beta=0.0, warmup=5, reps=20, flush_bytes=0, wall=False):
+        """(execute ms, SpMV kernel ms) per execute from HIP events; with wall=True also the
+        host wall-clock ms per execute over the same back-to-back run."""
+        out = (C.c_double * 3)()
         check(lib().spx_plan_time(self.h, alpha, x_dev.ptr, beta, y_dev.ptr, warmup, reps, flush_bytes, out))
-        return out[0], out[1]
+        return (out[0], out[1], out[2]) if wall else (out[0], out[1])
 
     def close(self):
         if getattr(self, "h", None):
