@@ -13,13 +13,13 @@
 // hits LDS; a unit flagged SEG_PARTIAL stores its segment sums into a
 // partial array instead of updating y.
 //
-// Per chunk of kBlock*8 nonzeros, each thread splits its 8 products at its
+// Per chunk of kSegBlock*8 nonzeros, each thread splits its 8 products at its
 // start bits into pieces.  Pieces bounded on both sides inside the thread are
 // whole segments and are written out at once.  The piece that continues from
 // earlier threads and the piece that runs on past the thread are joined by a
-// block-wide segmented scan (wave shuffles, then one exchange of the four
-// wave totals through LDS), which also counts start bits to give every piece
-// its segment index.  The sum still open at the end of a chunk carries into
+// block-wide segmented scan (DPP steps across each wave, then one exchange of
+// the four wave totals through LDS), which also counts start bits to give
+// every piece its segment index.  The sum still open at the end of a chunk carries into
 // the next chunk in registers.  One barrier per chunk: the wave totals are
 // double-buffered.  Every addition happens in a fixed order, so the result
 // is bitwise reproducible.
@@ -64,6 +64,26 @@ __device__ inline int seg_unit(const int32_t *xr, int iter) {
   return u < xr[x + 1] ? u : -1;
 }
 
+// One step of a wave64 inclusive scan with DPP (data-parallel primitives:
+// cross-lane operands read by the ALU, no LDS traffic).  Each lane combines
+// the (sum, closed, count) triple of the lane CTRL names, or the identity
+// (0, 0, 0) where that lane is out of range or masked off (old = 0 and
+// bound_ctrl off): the segmented sum restarts after a closing lane, the
+// count always adds.  The six steps row_shr 1, 2, 4, 8 then row_bcast 15
+// and 31 give the full 64-lane inclusive scan.
+template <int CTRL, int ROWS, int BANKS>
+__device__ inline void scan_step(double &v, int &sg, int &cn) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, BANKS, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, BANKS, false);
+  const int su = __builtin_amdgcn_update_dpp(0, sg, CTRL, ROWS, BANKS, false);
+  const int cu = __builtin_amdgcn_update_dpp(0, cn, CTRL, ROWS, BANKS, false);
+  cn += cu;
+  if (!sg) {
+    v += __hiloint2double(hi, lo);
+    sg |= su;
+  }
+}
+
 template <typename V, typename X>
 __device__ inline void seg_emit(const SegArgs<V, X> &a, bool part, int o, double s) {
   if (part)
@@ -73,10 +93,10 @@ __device__ inline void seg_emit(const SegArgs<V, X> &a, bool part, int o, double
 }
 
 template <typename V, typename X, bool STAGE>
-__global__ __launch_bounds__(kBlock) void k_seg(SegArgs<V, X> a) {
+__global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
   constexpr int LE = 8;
-  constexpr int CH = kBlock * LE;
-  constexpr int NW = kBlock / kWave;
+  constexpr int CH = kSegBlock * LE;
+  constexpr int NW = kSegBlock / kWave;
   __shared__ X s_x[STAGE ? kSegStage : 1];
   __shared__ double s_wv[2][NW];  // per wave: scan value, closed flag, start-bit count
   __shared__ int s_ws[2][NW];
@@ -91,16 +111,16 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs<V, X> a) {
     const bool part = (U.flags & SEG_PARTIAL) != 0;
     if (stg) {
       __syncthreads();  // the previous unit's gathers from s_x are done
-      for (int k0 = 0; k0 < U.cwin; k0 += 4 * kBlock) {
+      for (int k0 = 0; k0 < U.cwin; k0 += 4 * kSegBlock) {
         X xv[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          const int k = k0 + tid + i * kBlock;
+          const int k = k0 + tid + i * kSegBlock;
           if (k < U.cwin) xv[i] = a.x[U.cmin + k];
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          const int k = k0 + tid + i * kBlock;
+          const int k = k0 + tid + i * kSegBlock;
           if (k < U.cwin) s_x[k] = xv[i];
         }
       }
@@ -159,22 +179,15 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs<V, X> a) {
       const double run = nv > 0 && !tclosed ? last_val : 0.0;
       const int cnt = __builtin_popcount(fb);
       // segmented inclusive scan c(t) = run(t) + (has_close(t) ? 0 : c(t-1)),
-      // and an inclusive scan of the start-bit counts
+      // and an inclusive scan of the start-bit counts, across the wave
       double v = run;
       int sg = has_close, cn = cnt;
-#pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        const double vu = __shfl_up(v, off, kWave);
-        const int su = __shfl_up(sg, off, kWave);
-        const int cu = __shfl_up(cn, off, kWave);
-        if (lane >= off) {
-          cn += cu;
-          if (!sg) {
-            v += vu;
-            sg |= su;
-          }
-        }
-      }
+      scan_step<0x111, 0xf, 0xf>(v, sg, cn);  // row_shr:1
+      scan_step<0x112, 0xf, 0xf>(v, sg, cn);  // row_shr:2
+      scan_step<0x114, 0xf, 0xe>(v, sg, cn);  // row_shr:4
+      scan_step<0x118, 0xf, 0xc>(v, sg, cn);  // row_shr:8
+      scan_step<0x142, 0xa, 0xf>(v, sg, cn);  // row_bcast:15 into rows 1, 3
+      scan_step<0x143, 0xc, 0xf>(v, sg, cn);  // row_bcast:31 into rows 2, 3
       if (lane == kWave - 1) {
         s_wv[par][wid] = v;
         s_ws[par][wid] = sg;
@@ -228,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs<V, X> a) {
 template <typename K>
 static int seg_cap(K kern) {
   int nb = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kSegBlock, 0) != hipSuccess || nb < 1) nb = 1;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
     cus = 256;
@@ -242,7 +255,7 @@ static hipError_t seg_launch(const SegArgs<V, X> &a, hipStream_t st) {
   static const int cap = seg_cap(k_seg<V, X, STAGE>);
   int g = direct_grid(a.nunits);
   if (g > cap) g = cap;
-  hipLaunchKernelGGL((k_seg<V, X, STAGE>), dim3(g), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL((k_seg<V, X, STAGE>), dim3(g), dim3(kSegBlock), 0, st, a);
   return hipGetLastError();
 }
 

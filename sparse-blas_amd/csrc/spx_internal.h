@@ -63,7 +63,8 @@ struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
 };
 
 // ---- segmented engine (seg.hip) ----------------------------------------------
-constexpr int kSegStage = 3968;  // most x entries a unit stages in LDS (5 workgroups fit a CU)
+constexpr int kSegStage = 3968;  // most x entries a unit stages in LDS (31 KiB)
+constexpr int kSegBlock = 256;   // threads per segmented-engine workgroup (4 wave64s)
 
 // One workgroup unit of the segmented engine: padded elements [e0, e1)
 // (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order, with
