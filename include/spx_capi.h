@@ -81,6 +81,10 @@ typedef struct spx_plan_desc {
 /* desc.flags: the row pointer, column and value arrays are DEVICE memory
  * (the plan copies them; the caller keeps ownership). */
 #define SPX_FLAG_DEVICE_PTRS 1
+/* desc.flags: kernel-4 FP64 plans run spx_plan_execute_multi as fused passes
+ * of up to 32 columns (A read once per pass, a lane group per row); without
+ * it every column is a separate execute. */
+#define SPX_FLAG_FUSED_SPMM 2
 
 typedef struct spx_plan *spx_plan_t;
 
@@ -100,7 +104,9 @@ int spx_plan_execute(spx_plan_t p, double alpha, const double *x, double beta,
                      double *y, void *stream);
 
 /* Multi-RHS: Y(:,j) = alpha*A*X(:,j) + beta*Y(:,j) for j < k, X (n-by-k) and
- * Y (m-by-k) column-major with leading dimensions ldx >= n, ldy >= m. */
+ * Y (m-by-k) column-major with leading dimensions ldx >= n, ldy >= m (see
+ * SPX_FLAG_FUSED_SPMM).  The fused passes keep a plan-owned scratch of
+ * n * 32 doubles (allocated on first use). */
 int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
                            int64_t ldx, double beta, double *Y, int64_t ldy,
                            void *stream);

@@ -111,6 +111,11 @@ struct spx_plan {
   SegUnit *d_sunits = nullptr;
   int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work
   uint8_t *d_sstart = nullptr;
+  int32_t *d_sbeg = nullptr;  // per segment: first element (multi-RHS path)
+  double *d_xr = nullptr;     // multi-RHS scratch: X columns row-major
+  double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials
+  int64_t nxr = 0, npartk = 0;
+  bool fused_spmm = false;
   int32_t *d_scol = nullptr;
   double *d_sval = nullptr;
   float *d_sval32 = nullptr;
@@ -134,7 +139,7 @@ struct spx_plan {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
-                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_lsp, (void *)d_lseg,
+                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_sbeg, (void *)d_xr, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
                     (void *)d_scol, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
@@ -199,7 +204,7 @@ static int64_t long_threshold(const spx_plan_desc *d) {
 // units, nonzeros copied 8-aligned per unit, one start bit per element.
 struct SegHost {
   std::vector<SegUnit> units;
-  std::vector<int32_t> out, col;
+  std::vector<int32_t> out, sbeg, col;  // per segment: output, first element
   std::vector<double> val;
   std::vector<uint8_t> start;
   int64_t pos = 0;  // next free element
@@ -272,6 +277,7 @@ static void seg_append(SegHost &H, int64_t nseg, F seg, G brk, const int32_t *sc
       window(b, e, lo, hi);
       H.start[(size_t)(at >> 3)] |= (uint8_t)(1u << (at & 7));
       H.out.push_back(o);
+      H.sbeg.push_back((int32_t)at);
       H.col.insert(H.col.end(), scol + b, scol + e);
       H.val.insert(H.val.end(), sval + b, sval + e);
       at += e - b;
@@ -385,6 +391,7 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
   if ((rc = P->upload(&P->d_sxr, xr.data(), kXcd + 1))) return rc;
   if ((rc = P->upload(&P->d_sstart, H.start.data(), (int64_t)H.start.size()))) return rc;
+  if ((rc = P->upload(&P->d_sbeg, H.sbeg.data(), (int64_t)H.sbeg.size()))) return rc;
   if ((rc = P->upload(&P->d_scol, H.col.data(), total))) return rc;
   if (P->precision == SPX_PREC_F32) {
     std::vector<float> v32(H.val.begin(), H.val.end());
@@ -413,6 +420,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   P->mode = d->mode;
   P->precision = d->precision;
   P->queues = d->queues > 0 ? d->queues : 1;
+  P->fused_spmm = (d->flags & SPX_FLAG_FUSED_SPMM) != 0;
   if (m == 0) return SPX_OK;
 
   // ---- host row pointer (validated: monotone, from 0 to nnz) ----
@@ -523,7 +531,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
 
   // ---- long rows: (column block, row) slices for the segmented engine ----
   SegHost H;
-  const int64_t seg_target = (int64_t)kBlock * 8 * (d->kernel == SPX_KERNEL_SEG && d->items > 0 ? d->items : 2);
+  const int64_t seg_target = (int64_t)kSegBlock * 8 * (d->kernel == SPX_KERNEL_SEG && d->items > 0 ? d->items : 2);
   if (d->kernel == SPX_KERNEL_SEG && (d->items < 0 || d->items > 32))
     return fail(SPX_EARG, "items (chunks per unit) must be in 1..32");
   if (P->nlong > 0) {
@@ -532,7 +540,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     if (P->long_nnz + 8 * (int64_t)P->nblocks * P->nlong + 64 >= INT32_MAX)
       return fail(SPX_ERANGE, "padded long-row nonzeros must stay below 2^31 per plan");
     std::vector<int32_t> lsp, lseg;
-    seg_long(H, lrp, lcol, lval, P->nlong, P->nblocks, std::max<int64_t>(seg_target, 8 * kBlock * 8), lsp, lseg);
+    seg_long(H, lrp, lcol, lval, P->nlong, P->nblocks, std::max<int64_t>(seg_target, 4 * kSegBlock * 8), lsp, lseg);
     P->nslong = (int64_t)H.units.size();
     P->nlseg = (int64_t)lseg.size();
     int rc;
@@ -544,7 +552,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
 
   // ---- kernel 4: the regular rows join the long slices in one launch ----
   if (d->kernel == SPX_KERNEL_SEG) {
-    P->items = (int)(seg_target / (kBlock * 8));
+    P->items = (int)(seg_target / (kSegBlock * 8));
     // empty rows of A (the long rows are written by the reduction)
     std::vector<char> is_long(m, 0);
     for (int r : long_rows) is_long[r] = 1;
@@ -968,6 +976,58 @@ int spx_plan_execute(spx_plan_t p, double alpha, const double *x, double beta, d
 int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, int64_t ldx, double beta, double *Y,
                            int64_t ldy, void *stream) {
   if (!p || k < 0 || ldx < p->n || ldy < p->m) return fail(SPX_EARG, "bad multi-RHS arguments");
+  if (p->m == 0 || k == 0) return SPX_OK;
+  if (!X || !Y) return fail(SPX_EARG, "null X or Y");
+  if (p->fused_spmm && p->kernel == SPX_KERNEL_SEG && p->precision == SPX_PREC_F64) {
+    // fused passes of up to 32 columns: A is read once per pass
+    hipStream_t st = (hipStream_t)stream;
+    const int K = spmm_width(std::min(k, 32));
+    if (p->nxr < (int64_t)p->n * K || (p->nlong > 0 && p->npartk < p->nlseg * K))
+      HIP_TRY(hipDeviceSynchronize());  // earlier passes may still read the scratch being replaced
+    if (p->nxr < (int64_t)p->n * K) {
+      if (p->d_xr) HIP_TRY(hipFree(p->d_xr));
+      p->d_xr = nullptr;
+      p->nxr = 0;
+      int rc = p->alloc(&p->d_xr, (int64_t)p->n * K);
+      if (rc) return rc;
+      p->nxr = (int64_t)p->n * K;
+    }
+    if (p->nlong > 0 && p->npartk < p->nlseg * K) {
+      if (p->d_partk) HIP_TRY(hipFree(p->d_partk));
+      p->d_partk = nullptr;
+      p->npartk = 0;
+      int rc = p->alloc(&p->d_partk, p->nlseg * K);
+      if (rc) return rc;
+      p->npartk = p->nlseg * K;
+    }
+    for (int j0 = 0; j0 < k; j0 += 32) {
+      SpmmArgs a = {};
+      a.units = p->d_sunits;
+      a.nunits = p->nsunits;
+      a.xr = p->d_sxr;
+      a.sbeg = p->d_sbeg;
+      a.col = p->d_scol;
+      a.val = p->d_sval;
+      a.X = X + (size_t)j0 * ldx;
+      a.ldx = ldx;
+      a.n = p->n;
+      a.kk = std::min(32, k - j0);
+      a.XR = p->d_xr;
+      a.alpha = alpha;
+      a.beta = beta;
+      a.Y = Y + (size_t)j0 * ldy;
+      a.ldy = ldy;
+      a.empty = p->d_empty;
+      a.nempty = p->nempty;
+      a.partial = p->d_partk;
+      a.nlong = p->nlong;
+      a.lsp = p->d_lsp;
+      a.lseg = p->d_lseg;
+      a.lrows = p->d_lrows;
+      HIP_TRY(launch_spmm(a, st));
+    }
+    return SPX_OK;
+  }
   for (int j = 0; j < k; j++) {
     int rc = execute_impl(p, alpha, X + (size_t)j * ldx, beta, Y + (size_t)j * ldy, (hipStream_t)stream, nullptr,
                           nullptr, false);

@@ -52,18 +52,6 @@ __device__ inline void load8i(const int32_t *p, int32_t *o) {
   o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
 }
 
-// Unit for this workgroup's iteration `iter`, or -1 when done.  XCD x owns
-// units [xr[x], xr[x+1]), ranges the host balanced by work, and its
-// workgroups walk them round-robin in order, so neighbouring units (which
-// share x entries) run at the same time in that XCD's L2.  No atomics: a
-// shared counter serialises at memory once hundreds of workgroups pop it.
-__device__ inline int seg_unit(const int32_t *xr, int iter) {
-  const int x = blockIdx.x % kXcd;
-  const int per = gridDim.x / kXcd;
-  const int u = xr[x] + (int)(blockIdx.x / kXcd) + iter * per;
-  return u < xr[x + 1] ? u : -1;
-}
-
 // One step of a wave64 inclusive scan with DPP (data-parallel primitives:
 // cross-lane operands read by the ALU, no LDS traffic).  Each lane combines
 // the (sum, closed, count) triple of the lane CTRL names, or the identity

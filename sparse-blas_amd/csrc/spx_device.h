@@ -61,4 +61,16 @@ __device__ inline int64_t next_unit(int *queue, int64_t iter, int64_t count, int
   return u;
 }
 
+// Unit for this workgroup's iteration `iter`, or -1 when done.  XCD x owns
+// units [xr[x], xr[x+1]), ranges the host balanced by work, and its
+// workgroups walk them round-robin in order, so neighbouring units (which
+// share x entries) run at the same time in that XCD's L2.  No atomics: a
+// shared counter serialises at memory once hundreds of workgroups pop it.
+__device__ inline int seg_unit(const int32_t *xr, int iter) {
+  const int x = blockIdx.x % kXcd;
+  const int per = gridDim.x / kXcd;
+  const int u = xr[x] + (int)(blockIdx.x / kXcd) + iter * per;
+  return u < xr[x + 1] ? u : -1;
+}
+
 }  // namespace spx

@@ -91,6 +91,29 @@ struct SegArgs {
   double *partial;
 };
 
+// One multi-RHS pass (spmm.hip): columns of X and Y starting at the given
+// pointers, kk <= 32 of them.
+struct SpmmArgs {
+  const SegUnit *units;
+  int64_t nunits;
+  const int32_t *xr;
+  const int32_t *sbeg;  // per segment: its first element in the padded arrays
+  const int32_t *col;
+  const double *val;
+  const double *X;
+  int64_t ldx;
+  int n, kk;
+  double *XR;  // n * spmm_width(kk) doubles
+  double alpha, beta;
+  double *Y;
+  int64_t ldy;
+  const int32_t *empty;  // rows of A without nonzeros (long rows excluded)
+  int64_t nempty;
+  double *partial;  // slices * spmm_width(kk) doubles
+  int nlong;
+  const int32_t *lsp, *lseg, *lrows;
+};
+
 template <typename V, typename X>
 struct KArgs {
   const Part *parts;
@@ -151,6 +174,11 @@ hipError_t launch_merge_f64(const KArgs<double, double> &a, int items, int grid,
 hipError_t launch_merge_f32(const KArgs<float, XPair> &a, int items, int grid, hipStream_t st);
 hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st);
 hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st);
+
+// Multi-RHS SpMM pass on the segmented engine's layout (spmm.hip);
+// spmm_width(kk) = kk rounded up to a power of two (the row-major XR width).
+int spmm_width(int kk);
+hipError_t launch_spmm(const SpmmArgs &a, hipStream_t st);
 
 // Segmented engine: `stage` = some unit stages x in LDS (SEG_STAGE).
 hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st);

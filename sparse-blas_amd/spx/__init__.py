@@ -18,6 +18,7 @@ GEN_PATH = os.path.join(_PKG, "libspxgen.so")
 OK, EARG, EMEM, EDEV, EGUARD, ERANGE = 0, -1, -2, -3, -4, -5
 MODE_ROWS, MODE_NNZ, MODE_QUEUE = 0, 1, 2
 KERNEL_VECTOR, KERNEL_MERGE, KERNEL_TILE, KERNEL_SEG = 1, 2, 3, 4
+FLAG_DEVICE_PTRS, FLAG_FUSED_SPMM = 1, 2
 PREC_F64, PREC_F32 = 0, 1
 FLAG_DEVICE_PTRS = 1
 
@@ -175,7 +176,7 @@ class Plan:
     """Device-resident SpMV plan (spx_plan_create / execute / destroy)."""
 
     def __init__(self, rowptr, col, val, n=None, kernel=KERNEL_MERGE, mode=MODE_NNZ, parts=1, nb=0,
-                 queues=1, sigma=0, lanes=0, items=0, precision=PREC_F64, long_row=0):
+                 queues=1, sigma=0, lanes=0, items=0, precision=PREC_F64, long_row=0, flags=0):
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
         self.col = np.ascontiguousarray(col, dtype=np.int32)
         self.val = np.ascontiguousarray(val, dtype=np.float64)
@@ -191,6 +192,7 @@ class Plan:
         d.kernel, d.mode, d.parts, d.nb, d.queues = kernel, mode, parts, nb, queues
         d.sigma, d.lanes, d.items, d.precision = sigma, lanes, items, precision
         d.long_row = long_row
+        d.flags = flags
         h = C.c_void_p()
         check(lib().spx_plan_create(C.byref(h), C.byref(d)))
         self.h = h
