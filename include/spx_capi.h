@@ -174,6 +174,46 @@ int spx_graph_destroy(spx_graph_t g);
  * (events around the whole run), after `warmup` replays. */
 int spx_graph_time(spx_graph_t g, int warmup, int reps, double *out_ms);
 
+/* ---- learned schedule selector -------------------------------------------------
+ * No reference counterpart: it replaces the exhaustive (d, c) search of the
+ * reference harness (SB/test/dspmv_test.cu:314-332).  A schedule index s in
+ * [0, 96) means kernel 1 + s/24, tile width index (s/6)%4 and partition count
+ * {1, 2, 3, 4, 6, 8}[s%6]; widths are lanes {2, 8, 32, 64} (kernel 1), items
+ * {2, 4, 8, 16} (kernel 2), sigma {4, 8, 16, 32} (kernel 3), chunks per unit
+ * {1, 2, 4, 8} (kernel 4, which ignores the partition count).  All pointers
+ * below are device memory. */
+
+/* Parameters of the 32 -> 64 -> 64 -> 96 MLP (FP32, W1 b1 W2 b2 W3 b3 with
+ * W[out][in]). */
+int spx_sel_param_count(void);
+
+/* The 32 features of a CSR matrix (int32 row pointer and columns on the
+ * device), written to feat[32]; rows with at least long_row nonzeros count
+ * as long (0 = none).  Synchronous. */
+int spx_sel_features(int m, int n, int64_t nnz, const int32_t *rowptr,
+                     const int32_t *col, int64_t long_row, float *feat,
+                     void *stream);
+
+/* Floats of workspace the MLP entry points need for a batch of B samples. */
+int64_t spx_mlp_work_floats(int B);
+
+/* Scores Z[B][96] of feature rows X[B][32]. */
+int spx_mlp_forward(const float *params, const float *X, int B, float *Z,
+                    float *work, void *stream);
+
+/* One training step on a batch: forward, softmax cross-entropy against the
+ * target distributions T[B][96] (loss[B] per sample, may be NULL), gradient
+ * of the mean loss into grad[params], then an Adam step (lr, weight decay wd;
+ * step counts from 1) on params with moments m1, m2.  lr = 0 skips the
+ * update (gradient only). */
+int spx_mlp_train_step(float *params, float *m1, float *m2, int step, float lr,
+                       float wd, const float *X, const float *T, int B,
+                       float *loss, float *grad, float *work, void *stream);
+
+/* out[b] = argmax over allowed schedules (mask[o] != 0; NULL = all) of Z[b]. */
+int spx_mlp_argmax(const float *Z, int B, const int32_t *mask, int32_t *out,
+                   void *stream);
+
 /* ---- reference drop-in entry points (host pointers) ----------------------
  * Arguments and return codes as SparseBLAS/include/spmv_kernel.h:4-34. */
 int spx_dropin_baseline(int m, int n, long long nnz, double *alpha,

@@ -186,6 +186,12 @@ void spx_plan_desc_init(spx_plan_desc *d) {
 }
 
 const char *spx_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+void spx::set_last_error(const std::string &msg) { g_last_error = msg; }
+
+extern "C" {
 const char *spx_version(void) { return "spx 0.2 gfx950"; }
 
 }  // extern "C"

@@ -134,6 +134,9 @@ struct KArgs {
   int sigma;
 };
 
+// The thread's message for spx_last_error (plan.cpp).
+void set_last_error(const std::string &msg);
+
 // ---- launchers (kernels.hip) ---------------------------------------------
 enum { PART_NNZ = 0, PART_TASKS = 1, PART_ROWS = 2 };
 

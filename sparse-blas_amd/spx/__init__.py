@@ -52,6 +52,12 @@ CAPI = {
     "spx_plan_info": (_I, [_P, _P, _I]),
     "spx_plan_export": (_I64, [_P, _I, _P, _I64]),
     "spx_plan_time": (_I, [_P, _D, _P, _D, _P, _I, _I, _I64, _P]),
+    "spx_sel_param_count": (_I, []),
+    "spx_sel_features": (_I, [_I, _I, _I64, _P, _P, _I64, _P, _P]),
+    "spx_mlp_work_floats": (_I64, [_I]),
+    "spx_mlp_forward": (_I, [_P, _P, _I, _P, _P, _P]),
+    "spx_mlp_train_step": (_I, [_P, _P, _P, _I, C.c_float, C.c_float, _P, _P, _I, _P, _P, _P, _P]),
+    "spx_mlp_argmax": (_I, [_P, _I, _P, _P, _P]),
     "spx_dropin_baseline": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P, _P, _I]),
     "spx_dropin_v1": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P, _P, _I, _I]),
     "spx_dropin_v2": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P, _P, _I, _I, _LL, _I]),
