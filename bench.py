@@ -48,6 +48,109 @@ def profiled_traffic(kernel, items, window):
     return None
 
 
+def run_batch(a):
+    """The batched form on 512 uneven matrices (n ~ lognormal(ln 14142, 1.32)
+    clipped to [1e3, 2e5], Zipf(1.3) rows on [1, min(2048, n)], values
+    U[-1,1)): the plain form (one FP64 plan per matrix, the drop-in default
+    schedule: kernel 2, 8 items, one partition, one execute per matrix)
+    against the learned-selector + FP32-storage batched form (all matrices
+    in one plan, schedule from the selector, one hipGraph replay per batch).
+    Prints one JSON line: matrices/s of both, the ratio, and the FP32 form's
+    normwise error against the oracle on sampled members."""
+    import ctypes as C
+
+    import spx
+    from spx import selector as S
+    from spx import workloads as W
+
+    t0 = time.time()
+    mats = W.batch(a.batch_count, seed=5)
+    M = sum(A["m"] for A in mats)
+    NNZ = sum(A["nnz"] for A in mats)
+    log(f"batch: {len(mats)} matrices, rows {M}, nnz {NNZ} (generated in {time.time() - t0:.1f}s)")
+    x = W.vector(M, 23)
+    off = np.zeros(len(mats) + 1, np.int64)
+    np.cumsum([A["m"] for A in mats], out=off[1:])
+    xd, yd = spx.DeviceBuffer.from_array(x), spx.DeviceBuffer(M * 8)
+    L = spx.lib()
+
+    # plain form: per-matrix plans, default schedule, FP64
+    t0 = time.time()
+    plans = [spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=2) for A in mats]
+    log(f"plain: {len(plans)} plans built in {time.time() - t0:.1f}s")
+
+    def plain_round():
+        for i, P in enumerate(plans):
+            spx.check(L.spx_plan_execute(P.h, 1.0, C.c_void_p(xd.ptr + 8 * int(off[i])), 0.0,
+                                         C.c_void_p(yd.ptr + 8 * int(off[i])), None))
+
+    for _ in range(a.warmup):
+        plain_round()
+    spx.synchronize()
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        plain_round()
+    spx.synchronize()
+    plain_ms = (time.perf_counter() - t) * 1e3 / a.steps
+    for P in plans:
+        P.close()
+    del plans
+    log(f"plain: {plain_ms:.3f} ms per batch")
+
+    # learned selector on the packed block-diagonal matrix
+    t0 = time.time()
+    rp = np.zeros(M + 1, np.int64)
+    nz_off = np.cumsum([0] + [A["nnz"] for A in mats])
+    for i, A in enumerate(mats):
+        rp[off[i] + 1:off[i + 1] + 1] = A["rowptr"][1:] + nz_off[i]
+    col = np.concatenate([A["col"] + np.int32(off[i]) for i, A in enumerate(mats)])
+    sched, kw = S.select(dict(rowptr=rp, col=col, m=M, n=M))
+    del col, rp
+    log(f"selector: schedule {sched} {kw} ({time.time() - t0:.1f}s)")
+
+    out = {}
+    for prec, tag in ((spx.PREC_F32, "fp32"), (spx.PREC_F64, "fp64")):
+        t0 = time.time()
+        bkw = dict(kw)
+        if "parts" in bkw:  # kernels 1-3: the selector's partition count over nnz-balanced streams
+            bkw["mode"] = spx.MODE_NNZ
+        bp = spx.BatchPlan(mats, precision=prec, **bkw)
+        g = spx.Graph(bp, xd, yd)
+        ms = g.time(warmup=a.warmup, reps=a.steps)
+        log(f"batched {tag}: {ms:.3f} ms per batch (plan {time.time() - t0:.1f}s)")
+        y = yd.download(np.float64, M)
+        out[tag] = (ms, y)
+        g.close()
+        bp.close()
+
+    import oracle  # the checker: sampled members against the CPU oracle
+    rng = np.random.default_rng(0)
+    errs = []
+    for i in sorted(rng.choice(len(mats), min(16, len(mats)), replace=False)):
+        A = mats[i]
+        ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x[off[i]:off[i + 1]], 1.0, 0.0)
+        yi = out["fp32"][1][off[i]:off[i + 1]]
+        errs.append(float(np.linalg.norm(yi - ref) / np.linalg.norm(ref)))
+    B32 = sum(W.bytes_model(A["m"], A["n"], A["nnz"], 4) for A in mats)
+    B64 = sum(W.bytes_model(A["m"], A["n"], A["nnz"], 8) for A in mats)
+    ms32, ms64 = out["fp32"][0], out["fp64"][0]
+    res = {
+        "metric": "batched matrices/s: learned selector + FP32 storage + one graph vs plain FP64 per-matrix",
+        "value": round(len(mats) / (ms32 * 1e-3), 1), "unit": "matrices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms32, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 storage, f64 accumulation", "data": "synthetic",
+        "config": {"workload": f"{len(mats)} CSR matrices, n lognormal(ln 14142, 1.32) in [1e3, 2e5], "
+                               f"Zipf(1.3) rows on [1, 2048], {NNZ} nnz total", "schedule": kw},
+        "plain_fp64_matrices_per_s": round(len(mats) / (plain_ms * 1e-3), 1),
+        "batched_fp64_matrices_per_s": round(len(mats) / (ms64 * 1e-3), 1),
+        "speedup_vs_plain": round(plain_ms / ms32, 3),
+        "fp32_GBs": round(B32 / (ms32 * 1e-3) / 1e9, 1), "fp64_batched_GBs": round(B64 / (ms64 * 1e-3) / 1e9, 1),
+        "plain_GBs": round(B64 / (plain_ms * 1e-3) / 1e9, 1),
+        "fp32_normwise_rel_err_max": max(errs), "fp32_normwise_rel_err_median": float(np.median(errs)),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,11 +167,18 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--workload", choices=["circuit", "batch"], default="circuit",
+                    help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain")
+    ap.add_argument("--batch-count", type=int, default=512)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build drives one MI355X; --gpus must be 1 (the multi-GPU "
                                    "design is described, not built)"}))
         sys.exit(2)
+
+    if a.workload == "batch":
+        run_batch(a)
+        return
 
     import spx
     from spx import workloads as W
