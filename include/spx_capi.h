@@ -174,6 +174,19 @@ int spx_graph_destroy(spx_graph_t g);
  * (events around the whole run), after `warmup` replays. */
 int spx_graph_time(spx_graph_t g, int warmup, int reps, double *out_ms);
 
+/* ---- Matrix Market input (host) ----------------------------------------------------
+ * Replaces the reference's loader (main.cu:164-319; SB/include/mmio.h:254-545;
+ * SB/test/dspmv_test.cu:101-136).  Reads a coordinate file into a 0-based CSR
+ * with an int64 row pointer, keeping the file's order within each row:
+ * pattern entries read as 1.0, integer entries as doubles, symmetric and
+ * hermitian files mirror each off-diagonal entry, skew-symmetric files mirror
+ * it negated.  The three arrays are malloc'd; release each with spx_mm_free.
+ * Returns 0, or -1 cannot open, -2 bad banner / unsupported format, -3
+ * complex, -4 bad size line, -5 bad entry, -6 out of memory. */
+int spx_mm_read(const char *path, int *m, int *n, int64_t *nnz,
+                int64_t **rowptr, int32_t **col, double **val);
+void spx_mm_free(void *p);
+
 /* ---- learned schedule selector -------------------------------------------------
  * No reference counterpart: it replaces the exhaustive (d, c) search of the
  * reference harness (SB/test/dspmv_test.cu:314-332).  A schedule index s in

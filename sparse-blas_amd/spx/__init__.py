@@ -52,6 +52,8 @@ CAPI = {
     "spx_plan_info": (_I, [_P, _P, _I]),
     "spx_plan_export": (_I64, [_P, _I, _P, _I64]),
     "spx_plan_time": (_I, [_P, _D, _P, _D, _P, _I, _I, _I64, _P]),
+    "spx_mm_read": (_I, [C.c_char_p, _P, _P, _P, _P, _P, _P]),
+    "spx_mm_free": (None, [_P]),
     "spx_sel_param_count": (_I, []),
     "spx_sel_features": (_I, [_I, _I, _I64, _P, _P, _I64, _P, _P]),
     "spx_mlp_work_floats": (_I64, [_I]),
@@ -360,6 +362,27 @@ def csr5_kernel(m, n, nnz, alpha, val_dev, rowptr32_dev, col_dev, x_dev, beta, y
     a, b = C.c_double(alpha), C.c_double(beta)
     return lib().spx_dropin_csr5_kernel(m, n, nnz, C.addressof(a), val_dev.ptr, rowptr32_dev.ptr, col_dev.ptr,
                                         x_dev.ptr, C.addressof(b), y_dev.ptr)
+
+
+def read_mtx(path):
+    """A Matrix Market coordinate file as a CSR dict (rowptr int64, col int32,
+    val float64, m, n, nnz) via spx_mm_read; raises SpxError on a bad file."""
+    m, n, nnz = C.c_int(), C.c_int(), C.c_int64()
+    rp, col, val = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    rc = lib().spx_mm_read(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), C.byref(rp), C.byref(col),
+                           C.byref(val))
+    if rc != 0:
+        raise SpxError(rc, f"spx_mm_read({path!r}) failed with {rc}")
+    try:
+        M, Z = m.value, nnz.value
+        rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int64)), (M + 1,)).copy()
+        c = np.ctypeslib.as_array(C.cast(col, C.POINTER(C.c_int32)), (max(Z, 1),))[:Z].copy()
+        v = np.ctypeslib.as_array(C.cast(val, C.POINTER(C.c_double)), (max(Z, 1),))[:Z].copy()
+    finally:
+        for p in (rp, col, val):
+            lib().spx_mm_free(p)
+    from .workloads import finish
+    return finish(rowptr, c, v, m.value, n.value)
 
 
 def get_row_from_index(rowptr, idx):
