@@ -1,8 +1,8 @@
 """Build libspx.so (gfx950 HIP kernels + C ABI + drop-in C++ API) and
 libspxgen.so (host-only synthetic matrix generators) in-tree.
 
-    python sparse-blas_amd/build.py          # both libraries
-    python sparse-blas_amd/build.py --lib    # libspx.so only
+    python sparse-blas_amd/build.py          # both libraries and the test_spmv driver
+    python sparse-blas_amd/build.py --lib    # libspx.so (and the driver)
 
 hipcc cross-compiles for gfx950 without a GPU.  Packed FP32 instructions are
 disabled for all device code (-packed-fp32-ops).
@@ -69,6 +69,20 @@ def build_gen():
     return GENLIB
 
 
+HARNESS = os.path.join(HERE, "test_spmv")
+
+
+def build_harness():
+    """The reference's test driver workflow (csrc/test_spmv.cpp) linked
+    against libspx.so and libspxgen.so, found next to the binary at run time."""
+    tmp = HARNESS + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, os.path.join(CSRC, "test_spmv.cpp"),
+          "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-L", HERE, "-lspx", "-lspxgen",
+          "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HARNESS)
+    return HARNESS
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true")
@@ -79,6 +93,8 @@ def main(argv=None):
         print("built", build_gen())
     if a.lib or both:
         print("built", build_lib())
+    if both or (os.path.exists(LIB) and os.path.exists(GENLIB)):
+        print("built", build_harness())
 
 
 if __name__ == "__main__":
