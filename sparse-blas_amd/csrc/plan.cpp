@@ -393,6 +393,28 @@ static int seg_upload(spx_plan *P, SegHost &H) {
     xr[x] = (int32_t)(std::lower_bound(wsum.begin(), wsum.end(), wsum.back() * x / kXcd) - wsum.begin());
   xr[kXcd] = (int32_t)H.units.size();
   for (int x = 1; x <= kXcd; x++) xr[x] = std::max(xr[x], xr[x - 1]);
+  // the device order of the columns and values (lane-fastest per wave block)
+  {
+    std::vector<int32_t> pc((size_t)total, 0);
+    std::vector<double> pv((size_t)total, 0.0);
+    for (const SegUnit &U : H.units) {
+      const int64_t e8 = ((int64_t)U.e1 + 7) & ~(int64_t)7;
+      for (int64_t e = U.e0; e < e8; e++) {
+        pc[(size_t)seg_pos4(e, U.e0, U.e1)] = H.col[(size_t)e];
+        pv[(size_t)seg_pos8(e, U.e0, U.e1)] = H.val[(size_t)e];
+      }
+    }
+    if (P->precision == SPX_PREC_F32) {  // FP32 values follow the 4-byte order
+      std::vector<double> p4((size_t)total, 0.0);
+      for (const SegUnit &U : H.units) {
+        const int64_t e8 = ((int64_t)U.e1 + 7) & ~(int64_t)7;
+        for (int64_t e = U.e0; e < e8; e++) p4[(size_t)seg_pos4(e, U.e0, U.e1)] = H.val[(size_t)e];
+      }
+      pv.swap(p4);
+    }
+    H.col.swap(pc);
+    H.val.swap(pv);
+  }
   int rc;
   if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
   if ((rc = P->upload(&P->d_sxr, xr.data(), kXcd + 1))) return rc;

@@ -5,7 +5,8 @@
 // are grouped into units at segment boundaries, so no segment is ever split
 // between workgroups and no carries leave a unit.  Each unit's nonzeros are
 // copied to an 8-aligned region (padding <= 7 entries), so a thread owns 8
-// aligned nonzeros per chunk and fetches them with 16-byte loads, plus one
+// aligned nonzeros per chunk, stored lane-fastest per wave (seg_pos4,
+// seg_pos8) so its 16-byte loads are contiguous across the wave, plus one
 // byte of segment-start bits.  A unit's segments write consecutive outputs,
 // so a sum's destination follows from its index without a further load.  A
 // unit flagged SEG_STAGE first stages its
@@ -27,28 +28,38 @@
 
 namespace spx {
 
+// A lane's 8 elements of wave block B with L lanes (the layout of
+// seg_pos4 / seg_pos8): 16-byte loads whose runs are contiguous across lanes.
+// The matrix is read once per SpMV, so these loads are non-temporal: they do
+// not displace the x windows and y lines that other units reuse from L2.
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef int vi4 __attribute__((ext_vector_type(4)));
+
 template <typename V>
 struct Vec8;
 template <>
 struct Vec8<double> {
-  __device__ static inline void load(const double *p, double *o) {
+  __device__ static inline void load(const double *p, int64_t B, int L, int lane, double *o) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const double2 u = reinterpret_cast<const double2 *>(p)[i];
-      o[2 * i] = u.x;
-      o[2 * i + 1] = u.y;
+    for (int k = 0; k < 4; k++) {
+      const vd2 u = __builtin_nontemporal_load(reinterpret_cast<const vd2 *>(p + B + (int64_t)k * 2 * L + 2 * lane));
+      o[2 * k] = u.x;
+      o[2 * k + 1] = u.y;
     }
   }
 };
 template <>
 struct Vec8<float> {
-  __device__ static inline void load(const float *p, float *o) {
-    const float4 u = reinterpret_cast<const float4 *>(p)[0], w = reinterpret_cast<const float4 *>(p)[1];
+  __device__ static inline void load(const float *p, int64_t B, int L, int lane, float *o) {
+    const vf4 u = __builtin_nontemporal_load(reinterpret_cast<const vf4 *>(p + B + 4 * lane));
+    const vf4 w = __builtin_nontemporal_load(reinterpret_cast<const vf4 *>(p + B + (int64_t)4 * L + 4 * lane));
     o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
   }
 };
-__device__ inline void load8i(const int32_t *p, int32_t *o) {
-  const int4 u = reinterpret_cast<const int4 *>(p)[0], w = reinterpret_cast<const int4 *>(p)[1];
+__device__ inline void load8i(const int32_t *p, int64_t B, int L, int lane, int32_t *o) {
+  const vi4 u = __builtin_nontemporal_load(reinterpret_cast<const vi4 *>(p + B + 4 * lane));
+  const vi4 w = __builtin_nontemporal_load(reinterpret_cast<const vi4 *>(p + B + (int64_t)4 * L + 4 * lane));
   o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
 }
 
@@ -89,7 +100,11 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
   __shared__ double s_wv[2][NW];  // per wave: scan value, closed flag, start-bit count
   __shared__ int s_ws[2][NW];
   __shared__ int s_wc[2][NW];
+  __shared__ double s_out[kSegOut];  // sums closed in the current chunk, by output
+  __shared__ uint8_t s_ok[kSegOut];  // which of them are set
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+  for (int t = tid; t < kSegOut; t += kSegBlock) s_ok[t] = 0;
+  __syncthreads();
   int par = 0;  // double-buffer slot of the wave totals, alternating per chunk
   for (int iter = 0;; iter++) {
     const int u = seg_unit(a.xr, iter);
@@ -125,12 +140,14 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
       {
         int32_t ci[LE];
         V vv[LE];
-        if (nv > 0) {  // the arrays are padded past every unit: a group of 8 is readable
-          load8i(a.col + ea, ci);
-          Vec8<V>::load(a.val + ea, vv);
-          fb = a.start[ea >> 3];
+        if (nv > 0) {  // groups of 8 are padded: a lane's 8 slots are always readable
+          const int B = c + wid * kSegWaveBlock;  // this wave's block
+          const int L = seg_lanes(B, U.e1);
+          load8i(a.col, B, L, lane, ci);
+          Vec8<V>::load(a.val, B, L, lane, vv);
+          fb = __builtin_nontemporal_load(&a.start[ea >> 3]);
           if (nv < LE) fb &= (1u << nv) - 1;
-          if (ea + nv < U.e1) tclosed = a.start[(ea + LE) >> 3] & 1;
+          if (ea + nv < U.e1) tclosed = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]) & 1;
         } else {
 #pragma unroll
           for (int i = 0; i < LE; i++) {
@@ -138,17 +155,21 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
             vv[i] = (V)0;
           }
         }
+        // every lane gathers (index 0 past the unit's end, always in range) so
+        // the 8 gathers issue back to back instead of each behind a branch
         if (stg) {
 #pragma unroll
           for (int i = 0; i < LE; i++) {
             const int cc = i < nv ? ci[i] - U.cmin : 0;  // in [0, cwin) for real nonzeros
-            p[i] = i < nv ? prod(vv[i], s_x, cc) : 0.0;
+            const double t = prod(vv[i], s_x, cc);
+            p[i] = i < nv ? t : 0.0;
           }
         } else {
 #pragma unroll
           for (int i = 0; i < LE; i++) {
             const int cc = i < nv ? ci[i] : 0;
-            p[i] = i < nv ? prod(vv[i], a.x, cc) : 0.0;
+            const double t = prod(vv[i], a.x, cc);
+            p[i] = i < nv ? t : 0.0;
           }
         }
       }
@@ -199,10 +220,23 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
       double carry_in = __shfl_up(c_incl, 1, kWave);
       if (lane == 0) carry_in = prev;
       const int excl = cb + cn - cnt;  // output of the first segment begun by this thread
+      // The sums closed in this chunk have outputs in [base - 1, totc): they
+      // go to LDS and leave in one coalesced pass, unless the chunk closes
+      // more than kSegOut of them (then each is stored where it closes).
+      const int kb = base - 1;
+      const bool via_lds = totc - kb <= kSegOut;  // uniform
+      auto put = [&](int k, double v) {
+        if (via_lds) {
+          s_out[k - kb] = v;
+          s_ok[k - kb] = 1;
+        } else {
+          seg_emit(a, part, k, v);
+        }
+      };
       if (nv > 0) {
         // piece 0 closes the segment open before this thread (or one that
         // begins at its first product)
-        if (has_close) seg_emit(a, part, excl + (int)(fb & 1) - 1, first_val + carry_in);
+        if (has_close) put(excl + (int)(fb & 1) - 1, first_val + carry_in);
         // pieces beginning at inner start bits
         double acc = 0.0;
         int open = 0, k = excl + (int)(fb & 1) - 1;
@@ -210,7 +244,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
         for (int i = 1; i < LE; i++) {
           if (i < nv) {
             if ((inner >> i) & 1) {
-              if (open) seg_emit(a, part, k, acc);
+              if (open) put(k, acc);
               k++;
               acc = 0.0;
               open = 1;
@@ -218,7 +252,18 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
             if (open) acc += p[i];
           }
         }
-        if (open && tclosed) seg_emit(a, part, k, acc);
+        if (open && tclosed) put(k, acc);
+      }
+      if (via_lds) {
+        // s_out is written only after the next chunk's barrier, which every
+        // thread reaches after this pass: one buffer suffices
+        __syncthreads();
+        for (int t = tid; t < totc - kb; t += kSegBlock) {
+          if (s_ok[t]) {
+            seg_emit(a, part, kb + t, s_out[t]);
+            s_ok[t] = 0;
+          }
+        }
       }
       carry = tot;
       base = totc;

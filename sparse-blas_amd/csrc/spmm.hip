@@ -60,15 +60,15 @@ __global__ __launch_bounds__(kBlock) void k_spmm(const SegUnit *__restrict__ uni
         double v[UN], xv[UN];
 #pragma unroll
         for (int i = 0; i < UN; i++) {
-          c[i] = col[q + i];
-          v[i] = val[q + i];
+          c[i] = col[seg_pos4(q + i, U.e0, U.e1)];
+          v[i] = val[seg_pos8(q + i, U.e0, U.e1)];
         }
 #pragma unroll
         for (int i = 0; i < UN; i++) xv[i] = XR[(int64_t)c[i] * K + j];
 #pragma unroll
         for (int i = 0; i < UN; i++) acc += v[i] * xv[i];
       }
-      for (; q < e; q++) acc += val[q] * XR[(int64_t)col[q] * K + j];
+      for (; q < e; q++) acc += val[seg_pos8(q, U.e0, U.e1)] * XR[(int64_t)col[seg_pos4(q, U.e0, U.e1)] * K + j];
       const int o = U.o0 + s;
       if (U.flags & SEG_PARTIAL) {
         partial[(int64_t)o * K + j] = acc;

@@ -63,8 +63,9 @@ struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
 };
 
 // ---- segmented engine (seg.hip) ----------------------------------------------
-constexpr int kSegStage = 3968;  // most x entries a unit stages in LDS (31 KiB)
+constexpr int kSegStage = 3328;  // most x entries a unit stages in LDS (26 KiB)
 constexpr int kSegBlock = 256;   // threads per segmented-engine workgroup (4 wave64s)
+constexpr int kSegOut = 520;     // sums per chunk gathered in LDS before they are stored
 
 // One workgroup unit of the segmented engine: padded elements [e0, e1)
 // (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order, with
@@ -76,6 +77,29 @@ struct SegUnit {
 };
 
 enum { SEG_STAGE = 1, SEG_PARTIAL = 2 };  // SegUnit.flags
+
+// Physical order of the segmented engine's column and value arrays.  From
+// e0, a unit's elements form wave blocks of 512 (the last one shorter); in
+// block B, lane l owns elements B + 8l .. B + 8l + 7, and with L the block's
+// lanes (its elements rounded up to 8, over 8) element B + 8l + i sits at
+// B + (i/4)*4L + 4l + i%4 in the 4-byte arrays (columns, FP32 values) and at
+// B + (i/2)*2L + 2l + i%2 in the FP64 values.  Each 16-byte load instruction
+// of a wave then reads one contiguous run of 64 x 16 bytes.
+constexpr int kSegWaveBlock = 512;
+__host__ __device__ inline int seg_lanes(int64_t B, int64_t e1) {
+  const int64_t L = (e1 - B + 7) >> 3;
+  return L > 64 ? 64 : (int)L;
+}
+__host__ __device__ inline int64_t seg_pos4(int64_t e, int64_t e0, int64_t e1) {
+  const int64_t B = e0 + ((e - e0) & ~(int64_t)(kSegWaveBlock - 1));
+  const int off = (int)(e - B), l = off >> 3, i = off & 7;
+  return B + (int64_t)(i >> 2) * 4 * seg_lanes(B, e1) + 4 * l + (i & 3);
+}
+__host__ __device__ inline int64_t seg_pos8(int64_t e, int64_t e0, int64_t e1) {
+  const int64_t B = e0 + ((e - e0) & ~(int64_t)(kSegWaveBlock - 1));
+  const int off = (int)(e - B), l = off >> 3, i = off & 7;
+  return B + (int64_t)(i >> 1) * 2 * seg_lanes(B, e1) + 2 * l + (i & 1);
+}
 
 template <typename V, typename X>
 struct SegArgs {
