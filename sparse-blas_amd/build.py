@@ -83,6 +83,18 @@ def build_harness():
     return HARNESS
 
 
+def build_handle_check():
+    """The persistent-handle check program (tests/handle_check.cpp): a C++
+    caller of include/anonymouslib_hip.h linked against libspx.so."""
+    out = os.path.join(ROOT, "tests", "handle_check")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, os.path.join(ROOT, "tests", "handle_check.cpp"),
+          "-I", os.path.join(ROOT, "include"), "-L", HERE, "-lspx",
+          "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN/../sparse-blas_amd"])
+    os.replace(tmp, out)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true")
@@ -95,6 +107,7 @@ def main(argv=None):
         print("built", build_lib())
     if both or (os.path.exists(LIB) and os.path.exists(GENLIB)):
         print("built", build_harness())
+        print("built", build_handle_check())
 
 
 if __name__ == "__main__":

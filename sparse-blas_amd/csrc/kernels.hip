@@ -304,16 +304,22 @@ __global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict
                                                         const int32_t *__restrict__ lseg,
                                                         const int32_t *__restrict__ rows, double alpha,
                                                         double beta, int assign, double *__restrict__ y) {
-  constexpr int U = 4;  // independent loads in flight per thread
+  // Up to U slices per thread per round: all their slot indices are loaded
+  // first, then all the partials, so a row of up to U*kBlock slices costs two
+  // dependent memory round trips.  A thread adds its slices in slot order.
+  constexpr int U = 16;
   __shared__ double s_w[kBlock / kWave];
   const int r = blockIdx.x, tid = threadIdx.x;
   if (r >= nlong) return;
   const int j0 = lsp[r], j1 = lsp[r + 1];
   double s = 0.0;
   for (int j = j0 + tid; j < j1; j += U * kBlock) {
+    int q[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) q[k] = j + k * kBlock < j1 ? lseg[j + k * kBlock] : -1;
     double v[U];
 #pragma unroll
-    for (int k = 0; k < U; k++) v[k] = j + k * kBlock < j1 ? partial[lseg[j + k * kBlock]] : 0.0;
+    for (int k = 0; k < U; k++) v[k] = q[k] >= 0 ? partial[q[k]] : 0.0;
 #pragma unroll
     for (int k = 0; k < U; k++) s += v[k];
   }
