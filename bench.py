@@ -164,6 +164,8 @@ def main():
                     help="half-width of the regular rows' column band (0 = uniform columns)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=100)
+    ap.add_argument("--cold-reps", type=int, default=20,
+                    help="executes timed with L2/MALL flushed before each (0 = skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -212,6 +214,13 @@ def main():
     log(f"timed {a.steps} steps: wall {wall_ms:.4f} ms/step, events {tot_ms:.4f} ms/execute, "
         f"kernel {ker_ms:.4f} ms")
 
+    # the same kernel with L2 and MALL cold: a 512 MiB buffer is overwritten
+    # before every timed execute (outside its events)
+    cold_ms = None
+    if a.cold_reps > 0:
+        _, cold_ms = plan.time(xd, yd, alpha, beta, warmup=1, reps=a.cold_reps, flush_bytes=512 << 20)
+        log(f"cold caches: kernel {cold_ms:.4f} ms ({a.cold_reps} executes, 512 MiB overwritten before each)")
+
     if a.copy_mb > 0:
         nb = a.copy_mb << 20
         src, dst = spx.DeviceBuffer(nb), spx.DeviceBuffer(nb)
@@ -251,6 +260,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(a.kernel, info["items"], a.window),
             "bytes_per_launch": B, "kernel_ms": round(ker_ms, 5),
+            "kernel_ms_cold_caches": None if cold_ms is None else round(cold_ms, 5),
         },
     }
 
@@ -270,9 +280,26 @@ def main():
             "sample": f"whole matrix, median of {a.cpu_reps} reps after 1 warm-up, 1 thread "
                       f"({t_cpu * 1e3:.1f} ms/SpMV); main.cu:350-356 loop restated in C",
         }
+        # the same loop with rows split statically over the host's threads
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        reps = []
+        for i in range(a.cpu_reps + 1):
+            t = time.perf_counter()
+            y_omp = oracle.spmv_omp(A["rowptr"], A["col"], A["val"], x, alpha, beta, threads=threads)
+            if i:
+                reps.append(time.perf_counter() - t)
+        t_omp = float(np.median(reps))
+        if not np.array_equal(y_omp, ref):
+            raise AssertionError("the OpenMP host loop differs from the sequential one")
+        out["cpu_baseline_openmp"] = {
+            "value": round(B / t_omp / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"whole matrix, median of {a.cpu_reps} reps after 1 warm-up, {threads} OpenMP threads, "
+                      f"static row split ({t_omp * 1e3:.1f} ms/SpMV)",
+        }
         err = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
         out["err_vs_oracle"] = err
-        log(f"cpu baseline {t_cpu * 1e3:.1f} ms/SpMV; normwise rel err vs oracle {err:.3e}")
+        log(f"cpu baseline {t_cpu * 1e3:.1f} ms/SpMV (1 thread), {t_omp * 1e3:.1f} ms/SpMV ({threads} threads); "
+            f"normwise rel err vs oracle {err:.3e}")
     print(json.dumps(out), flush=True)
 
 

@@ -41,6 +41,7 @@ def lib():
     L = C.CDLL(_LIB_PATH)
     sig = {
         "orc_spmv": (None, [C.c_int, i64p, i32p, f64p, f64p, C.c_double, C.c_double, f64p]),
+        "orc_spmv_omp": (None, [C.c_int, i64p, i32p, f64p, f64p, C.c_double, C.c_double, f64p, C.c_int]),
         "orc_spmv_f32val": (None, [C.c_int, i64p, i32p, f32p, f64p, C.c_double, C.c_double, f64p]),
         "orc_spmv_dot2": (None, [C.c_int, i64p, i32p, f64p, f64p, C.c_double, C.c_double, f64p, C.c_void_p]),
         "orc_spmm": (None, [C.c_int, C.c_int, C.c_int, i64p, i32p, f64p, f64p, C.c_double, C.c_double, f64p]),
@@ -82,6 +83,16 @@ def spmv(rowptr, col, val, x, alpha=1.0, beta=0.0, y=None):
     m = len(rowptr) - 1
     out = np.zeros(m) if y is None else np.array(y, dtype=np.float64, copy=True)
     lib().orc_spmv(m, rowptr, col, val, np.ascontiguousarray(x, np.float64), alpha, beta, out)
+    return out
+
+
+def spmv_omp(rowptr, col, val, x, alpha=1.0, beta=0.0, y=None, threads=1):
+    """The host loop with rows split statically over `threads` OpenMP threads
+    (bitwise equal to spmv)."""
+    rowptr, col, val = _csr(rowptr, col, val)
+    m = len(rowptr) - 1
+    out = np.zeros(m) if y is None else np.array(y, dtype=np.float64, copy=True)
+    lib().orc_spmv_omp(m, rowptr, col, val, np.ascontiguousarray(x, np.float64), alpha, beta, out, int(threads))
     return out
 
 

@@ -31,6 +31,22 @@ void orc_spmv(int m, const int64_t *rowptr, const int32_t *col,
     orc_spmv_rows(0, m, rowptr, col, val, x, alpha, beta, y);
 }
 
+void orc_spmv_omp(int m, const int64_t *rowptr, const int32_t *col,
+                  const double *val, const double *x, double alpha,
+                  double beta, double *y, int nthreads)
+{
+    /* main.cu:350-356 with the rows split statically over nthreads OpenMP
+     * threads: every row is summed exactly as orc_spmv sums it, so the
+     * result is bitwise the same for any thread count. */
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int i = 0; i < m; i++) {
+        double sum = 0;
+        for (int64_t j = rowptr[i]; j < rowptr[i + 1]; j++)
+            sum += x[col[j]] * val[j] * alpha;
+        y[i] = (beta == 0.0) ? sum : sum + beta * y[i];
+    }
+}
+
 void orc_spmv_f32val(int m, const int64_t *rowptr, const int32_t *col,
                      const float *val, const double *x, double alpha,
                      double beta, double *y)

@@ -27,6 +27,10 @@ extern "C" {
 void orc_spmv(int m, const int64_t *rowptr, const int32_t *col,
               const double *val, const double *x, double alpha, double beta,
               double *y);
+/* The same loop, rows split statically over nthreads OpenMP threads. */
+void orc_spmv_omp(int m, const int64_t *rowptr, const int32_t *col,
+                  const double *val, const double *x, double alpha,
+                  double beta, double *y, int nthreads);
 
 /* Same loop over FP32-stored values (the precision form's reference is still
  * FP64: each value is widened to double before the product). */

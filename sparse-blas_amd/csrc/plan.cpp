@@ -111,10 +111,8 @@ struct spx_plan {
   SegUnit *d_sunits = nullptr;
   int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work
   uint8_t *d_sstart = nullptr;
-  int32_t *d_sbeg = nullptr;  // per segment: first element (multi-RHS path)
-  double *d_xr = nullptr;     // multi-RHS scratch: X columns row-major
   double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials
-  int64_t nxr = 0, npartk = 0;
+  int64_t npartk = 0;
   bool fused_spmm = false;
   int32_t *d_scol = nullptr;
   double *d_sval = nullptr;
@@ -139,7 +137,7 @@ struct spx_plan {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
-                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_sbeg, (void *)d_xr, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
+                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
                     (void *)d_scol, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
@@ -210,7 +208,7 @@ static int64_t long_threshold(const spx_plan_desc *d) {
 // units, nonzeros copied 8-aligned per unit, one start bit per element.
 struct SegHost {
   std::vector<SegUnit> units;
-  std::vector<int32_t> out, sbeg, col;  // per segment: output, first element
+  std::vector<int32_t> out, col;  // per segment: output
   std::vector<double> val;
   std::vector<uint8_t> start;
   int64_t pos = 0;  // next free element
@@ -283,7 +281,6 @@ static void seg_append(SegHost &H, int64_t nseg, F seg, G brk, const int32_t *sc
       window(b, e, lo, hi);
       H.start[(size_t)(at >> 3)] |= (uint8_t)(1u << (at & 7));
       H.out.push_back(o);
-      H.sbeg.push_back((int32_t)at);
       H.col.insert(H.col.end(), scol + b, scol + e);
       H.val.insert(H.val.end(), sval + b, sval + e);
       at += e - b;
@@ -419,7 +416,6 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
   if ((rc = P->upload(&P->d_sxr, xr.data(), kXcd + 1))) return rc;
   if ((rc = P->upload(&P->d_sstart, H.start.data(), (int64_t)H.start.size()))) return rc;
-  if ((rc = P->upload(&P->d_sbeg, H.sbeg.data(), (int64_t)H.sbeg.size()))) return rc;
   if ((rc = P->upload(&P->d_scol, H.col.data(), total))) return rc;
   if (P->precision == SPX_PREC_F32) {
     std::vector<float> v32(H.val.begin(), H.val.end());
@@ -1007,20 +1003,11 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
   if (p->m == 0 || k == 0) return SPX_OK;
   if (!X || !Y) return fail(SPX_EARG, "null X or Y");
   if (p->fused_spmm && p->kernel == SPX_KERNEL_SEG && p->precision == SPX_PREC_F64) {
-    // fused passes of up to 32 columns: A is read once per pass
+    // fused passes of up to kSpmmPass columns: A is read once per pass
     hipStream_t st = (hipStream_t)stream;
-    const int K = spmm_width(std::min(k, 32));
-    if (p->nxr < (int64_t)p->n * K || (p->nlong > 0 && p->npartk < p->nlseg * K))
-      HIP_TRY(hipDeviceSynchronize());  // earlier passes may still read the scratch being replaced
-    if (p->nxr < (int64_t)p->n * K) {
-      if (p->d_xr) HIP_TRY(hipFree(p->d_xr));
-      p->d_xr = nullptr;
-      p->nxr = 0;
-      int rc = p->alloc(&p->d_xr, (int64_t)p->n * K);
-      if (rc) return rc;
-      p->nxr = (int64_t)p->n * K;
-    }
+    const int K = spmm_width(std::min(k, kSpmmPass));
     if (p->nlong > 0 && p->npartk < p->nlseg * K) {
+      HIP_TRY(hipDeviceSynchronize());  // earlier passes may still read the scratch being replaced
       if (p->d_partk) HIP_TRY(hipFree(p->d_partk));
       p->d_partk = nullptr;
       p->npartk = 0;
@@ -1028,19 +1015,17 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
       if (rc) return rc;
       p->npartk = p->nlseg * K;
     }
-    for (int j0 = 0; j0 < k; j0 += 32) {
+    for (int j0 = 0; j0 < k; j0 += kSpmmPass) {
       SpmmArgs a = {};
       a.units = p->d_sunits;
       a.nunits = p->nsunits;
       a.xr = p->d_sxr;
-      a.sbeg = p->d_sbeg;
+      a.start = p->d_sstart;
       a.col = p->d_scol;
       a.val = p->d_sval;
       a.X = X + (size_t)j0 * ldx;
       a.ldx = ldx;
-      a.n = p->n;
-      a.kk = std::min(32, k - j0);
-      a.XR = p->d_xr;
+      a.kk = std::min(kSpmmPass, k - j0);
       a.alpha = alpha;
       a.beta = beta;
       a.Y = Y + (size_t)j0 * ldy;

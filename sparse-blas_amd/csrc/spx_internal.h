@@ -117,19 +117,20 @@ struct SegArgs {
 
 // One multi-RHS pass (spmm.hip): columns of X and Y starting at the given
 // pointers, kk <= 32 of them.
+constexpr int kSpmmPass = 8;  // right-hand sides per fused SpMM pass (A read once per pass)
+
 struct SpmmArgs {
   const SegUnit *units;
   int64_t nunits;
   const int32_t *xr;
-  const int32_t *sbeg;  // per segment: its first element in the padded arrays
+  const uint8_t *start;  // segment-start bits, one byte per 8 elements
   const int32_t *col;
   const double *val;
-  const double *X;
+  const double *X;  // the pass's first column
   int64_t ldx;
-  int n, kk;
-  double *XR;  // n * spmm_width(kk) doubles
+  int kk;  // columns in this pass, 1..kSpmmPass
   double alpha, beta;
-  double *Y;
+  double *Y;  // the pass's first column
   int64_t ldy;
   const int32_t *empty;  // rows of A without nonzeros (long rows excluded)
   int64_t nempty;
@@ -203,7 +204,7 @@ hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t
 hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st);
 
 // Multi-RHS SpMM pass on the segmented engine's layout (spmm.hip);
-// spmm_width(kk) = kk rounded up to a power of two (the row-major XR width).
+// spmm_width(kk) = kk rounded up to a power of two (a pass's partial stride).
 int spmm_width(int kk);
 hipError_t launch_spmm(const SpmmArgs &a, hipStream_t st);
 

@@ -50,6 +50,20 @@ def test_spmv_exact_on_small_integers():
     assert np.array_equal(y, np.array(exact, dtype=float))
 
 
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_openmp_rows_equal_the_sequential_loop(threads):
+    # the CPU baseline's parallel form sums every row exactly as the
+    # sequential loop does, so the two agree bit for bit
+    rowptr, col, val, D = rand_csr(301, 120, 0.3, rng, empty_frac=0.1)
+    x = rng.uniform(-1, 1, 120)
+    y0 = rng.uniform(-1, 1, 301)
+    for alpha, beta in [(1.0, 0.0), (-0.5, 1.5)]:
+        ref = oracle.spmv(rowptr, col, val, x, alpha, beta, y0)
+        y = oracle.spmv_omp(rowptr, col, val, x, alpha, beta, y0, threads=threads)
+        assert np.array_equal(y, ref)
+    assert np.allclose(oracle.spmv_omp(rowptr, col, val, x, threads=threads), D @ x, rtol=1e-13, atol=1e-14)
+
+
 def test_beta_zero_never_reads_y():
     rowptr, col, val, D = rand_csr(10, 10, 0.5, rng)
     x = rng.uniform(-1, 1, 10)
