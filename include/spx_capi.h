@@ -82,8 +82,8 @@ typedef struct spx_plan_desc {
  * (the plan copies them; the caller keeps ownership). */
 #define SPX_FLAG_DEVICE_PTRS 1
 /* desc.flags: kernel-4 FP64 plans run spx_plan_execute_multi as fused passes
- * of up to 32 columns (A read once per pass, a lane group per row); without
- * it every column is a separate execute. */
+ * of up to 16 columns (A read once per pass; the lanes of each wave split into
+ * one group per column); without it every column is a separate execute. */
 #define SPX_FLAG_FUSED_SPMM 2
 
 typedef struct spx_plan *spx_plan_t;

@@ -111,10 +111,12 @@ struct spx_plan {
   SegUnit *d_sunits = nullptr;
   int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work
   uint8_t *d_sstart = nullptr;
+  double *d_xr = nullptr;     // multi-RHS scratch: the pass's X columns, row-major
   double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials
-  int64_t npartk = 0;
+  int64_t nxr = 0, npartk = 0;
   bool fused_spmm = false;
-  int32_t *d_scol = nullptr;
+  uint16_t *d_scol16 = nullptr;
+  int32_t *d_scol32 = nullptr;
   double *d_sval = nullptr;
   float *d_sval32 = nullptr;
   int64_t nsunits = 0, nslong = 0;  // units; the first nslong sweep long rows
@@ -137,8 +139,8 @@ struct spx_plan {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
-                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
-                    (void *)d_scol, (void *)d_sval, (void *)d_sval32})
+                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_xr, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
+                    (void *)d_scol16, (void *)d_scol32, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
     for (auto e : done) (void)hipEventDestroy(e);
@@ -390,15 +392,27 @@ static int seg_upload(spx_plan *P, SegHost &H) {
     xr[x] = (int32_t)(std::lower_bound(wsum.begin(), wsum.end(), wsum.back() * x / kXcd) - wsum.begin());
   xr[kXcd] = (int32_t)H.units.size();
   for (int x = 1; x <= kXcd; x++) xr[x] = std::max(xr[x], xr[x - 1]);
-  // the device order of the columns and values (lane-fastest per wave block)
+  // the device order of the columns and values: values lane-fastest per wave
+  // block; columns of staged units as 16-bit window offsets in element
+  // order (a lane's 8 are one 16-byte load), those of other units in full,
+  // lane-fastest, packed unit after unit into col32 from U.cmin on
+  std::vector<uint16_t> c16((size_t)total, 0);
+  std::vector<int32_t> c32;
   {
-    std::vector<int32_t> pc((size_t)total, 0);
     std::vector<double> pv((size_t)total, 0.0);
-    for (const SegUnit &U : H.units) {
+    for (SegUnit &U : H.units) {
       const int64_t e8 = ((int64_t)U.e1 + 7) & ~(int64_t)7;
-      for (int64_t e = U.e0; e < e8; e++) {
-        pc[(size_t)seg_pos4(e, U.e0, U.e1)] = H.col[(size_t)e];
-        pv[(size_t)seg_pos8(e, U.e0, U.e1)] = H.val[(size_t)e];
+      for (int64_t e = U.e0; e < e8; e++) pv[(size_t)seg_pos8(e, U.e0, U.e1)] = H.val[(size_t)e];
+      if (U.flags & SEG_STAGE) {
+        for (int64_t e = U.e0; e < U.e1; e++) c16[(size_t)e] = (uint16_t)(H.col[(size_t)e] - U.cmin);
+      } else {
+        const size_t base = c32.size();
+        if (base + (size_t)(e8 - U.e0) >= ((size_t)1 << 31)) return fail(SPX_ERANGE, "too many unstaged nonzeros");
+        c32.resize(base + (size_t)(e8 - U.e0), 0);
+        for (int64_t e = U.e0; e < e8; e++)
+          c32[base + (size_t)(seg_pos4(e, U.e0, U.e1) - U.e0)] = e < U.e1 ? H.col[(size_t)e] : 0;
+        U.cmin = (int32_t)base;
+        U.cwin = 0;
       }
     }
     if (P->precision == SPX_PREC_F32) {  // FP32 values follow the 4-byte order
@@ -409,14 +423,15 @@ static int seg_upload(spx_plan *P, SegHost &H) {
       }
       pv.swap(p4);
     }
-    H.col.swap(pc);
     H.val.swap(pv);
   }
+  std::vector<int32_t>().swap(H.col);
   int rc;
   if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
   if ((rc = P->upload(&P->d_sxr, xr.data(), kXcd + 1))) return rc;
   if ((rc = P->upload(&P->d_sstart, H.start.data(), (int64_t)H.start.size()))) return rc;
-  if ((rc = P->upload(&P->d_scol, H.col.data(), total))) return rc;
+  if ((rc = P->upload(&P->d_scol16, c16.data(), total))) return rc;
+  if (!c32.empty() && (rc = P->upload(&P->d_scol32, c32.data(), (int64_t)c32.size()))) return rc;
   if (P->precision == SPX_PREC_F32) {
     std::vector<float> v32(H.val.begin(), H.val.end());
     if ((rc = P->upload(&P->d_sval32, v32.data(), total))) return rc;
@@ -934,11 +949,11 @@ static KArgs<V, X> make_args(spx_plan *P, const V *val, const X *x, double alpha
 
 static hipError_t launch_seg_all(spx_plan *P, const double *x, double alpha, double beta, double *y, hipStream_t st) {
   if (P->precision == SPX_PREC_F32) {
-    SegArgs<float, XPair> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol, P->d_sval32, P->d_xhl,
+    SegArgs<float, XPair> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval32, P->d_xhl,
                                alpha, beta, y, P->d_partial};
     return launch_seg_f32(a, P->sstage, st);
   }
-  SegArgs<double, double> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol, P->d_sval, x,
+  SegArgs<double, double> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval, x,
                                alpha, beta, y, P->d_partial};
   return launch_seg_f64(a, P->sstage, st);
 }
@@ -1006,8 +1021,17 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
     // fused passes of up to kSpmmPass columns: A is read once per pass
     hipStream_t st = (hipStream_t)stream;
     const int K = spmm_width(std::min(k, kSpmmPass));
-    if (p->nlong > 0 && p->npartk < p->nlseg * K) {
+    if (p->nxr < (int64_t)p->n * K || (p->nlong > 0 && p->npartk < p->nlseg * K))
       HIP_TRY(hipDeviceSynchronize());  // earlier passes may still read the scratch being replaced
+    if (p->nxr < (int64_t)p->n * K) {
+      if (p->d_xr) HIP_TRY(hipFree(p->d_xr));
+      p->d_xr = nullptr;
+      p->nxr = 0;
+      int rc = p->alloc(&p->d_xr, (int64_t)p->n * K);
+      if (rc) return rc;
+      p->nxr = (int64_t)p->n * K;
+    }
+    if (p->nlong > 0 && p->npartk < p->nlseg * K) {
       if (p->d_partk) HIP_TRY(hipFree(p->d_partk));
       p->d_partk = nullptr;
       p->npartk = 0;
@@ -1021,11 +1045,14 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
       a.nunits = p->nsunits;
       a.xr = p->d_sxr;
       a.start = p->d_sstart;
-      a.col = p->d_scol;
+      a.col16 = p->d_scol16;
+      a.col32 = p->d_scol32;
       a.val = p->d_sval;
       a.X = X + (size_t)j0 * ldx;
       a.ldx = ldx;
+      a.n = p->n;
       a.kk = std::min(kSpmmPass, k - j0);
+      a.XR = p->d_xr;
       a.alpha = alpha;
       a.beta = beta;
       a.Y = Y + (size_t)j0 * ldy;

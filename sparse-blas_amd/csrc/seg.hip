@@ -88,7 +88,10 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
         if (nv > 0) {  // groups of 8 are padded: a lane's 8 slots are always readable
           const int B = c + wid * kSegWaveBlock;  // this wave's block
           const int L = seg_lanes(B, U.e1);
-          load8i(a.col, B, L, lane, ci);
+          if (stg)
+            load8u16(a.col16 + ea, ci);  // window offsets
+          else
+            load8i(a.col32 + U.cmin, B - U.e0, L, lane, ci);
           Vec8<V>::load(a.val, B, L, lane, vv);
           fb = __builtin_nontemporal_load(&a.start[ea >> 3]);
           if (nv < LE) fb &= (1u << nv) - 1;
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
         if (stg) {
 #pragma unroll
           for (int i = 0; i < LE; i++) {
-            const int cc = i < nv ? ci[i] - U.cmin : 0;  // in [0, cwin) for real nonzeros
+            const int cc = i < nv ? ci[i] : 0;  // in [0, cwin) for real nonzeros
             const double t = prod(vv[i], s_x, cc);
             p[i] = i < nv ? t : 0.0;
           }
@@ -118,19 +121,21 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
           }
         }
       }
-      // piece 0 = products before the first inner start bit; the last piece
-      // starts at the last start bit (piece 0 when there is none)
+      // Running sums that restart at every start bit: p[i] becomes the sum of
+      // the thread's products from the last start bit at or before i (or from
+      // its first product) through i.  Piece 0 is the run before the first
+      // inner start bit; the last piece's sum is p[nv-1].
       const uint32_t inner = fb & ~1u;
       const int f1 = inner ? __builtin_ctz(inner) : nv;
-      const int fl = inner ? 31 - __builtin_clz(inner) : 0;
-      double first_val = 0.0, last_val = 0.0;
 #pragma unroll
-      for (int i = 0; i < LE; i++) {
-        if (i < f1) first_val += p[i];
-        if (i >= fl && i < nv) last_val += p[i];
-      }
+      for (int i = 1; i < LE; i++)
+        if (!((fb >> i) & 1)) p[i] = p[i - 1] + p[i];
+      double last = p[0];
+#pragma unroll
+      for (int i = 1; i < LE; i++)
+        if (i < nv) last = p[i];
       const int has_close = nv > 0 && (inner != 0 || tclosed);
-      const double run = nv > 0 && !tclosed ? last_val : 0.0;
+      const double run = nv > 0 && !tclosed ? last : 0.0;
       const int cnt = __builtin_popcount(fb);
       // segmented inclusive scan c(t) = run(t) + (has_close(t) ? 0 : c(t-1)),
       // and an inclusive scan of the start-bit counts, across the wave
@@ -178,26 +183,15 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
           seg_emit(a, part, k, v);
         }
       };
-      if (nv > 0) {
-        // piece 0 closes the segment open before this thread (or one that
-        // begins at its first product)
-        if (has_close) put(excl + (int)(fb & 1) - 1, first_val + carry_in);
-        // pieces beginning at inner start bits
-        double acc = 0.0;
-        int open = 0, k = excl + (int)(fb & 1) - 1;
+      // Every element that ends a segment emits its running sum (plus the
+      // carry from earlier threads for piece 0) to the segment's output.
+      // ends: bit i set when element i is the last of its segment.
+      const uint32_t ends = nv > 0 ? ((fb >> 1) | ((uint32_t)tclosed << (nv - 1))) & ((1u << nv) - 1) : 0u;
+      int k = excl - 1;  // the output of the segment holding element i
 #pragma unroll
-        for (int i = 1; i < LE; i++) {
-          if (i < nv) {
-            if ((inner >> i) & 1) {
-              if (open) put(k, acc);
-              k++;
-              acc = 0.0;
-              open = 1;
-            }
-            if (open) acc += p[i];
-          }
-        }
-        if (open && tclosed) put(k, acc);
+      for (int i = 0; i < LE; i++) {
+        k += (fb >> i) & 1;
+        if ((ends >> i) & 1) put(k, i < f1 ? p[i] + carry_in : p[i]);
       }
       if (via_lds) {
         // s_out is written only after the next chunk's barrier, which every

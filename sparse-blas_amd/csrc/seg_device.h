@@ -41,6 +41,16 @@ __device__ inline void load8i(const int32_t *p, int64_t B, int L, int lane, int3
   o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
 }
 
+// 8 consecutive 16-bit column offsets (16-byte aligned), widened to int.
+__device__ inline void load8u16(const uint16_t *p, int32_t *o) {
+  const vi4 u = __builtin_nontemporal_load(reinterpret_cast<const vi4 *>(p));
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    o[2 * k] = (int32_t)((uint32_t)u[k] & 0xffffu);
+    o[2 * k + 1] = (int32_t)((uint32_t)u[k] >> 16);
+  }
+}
+
 // One step of a wave64 inclusive scan with DPP (data-parallel primitives:
 // cross-lane operands read by the ALU, no LDS traffic).  Each lane combines
 // the (sum, closed, count) triple of the lane CTRL names, or the identity

@@ -1,31 +1,48 @@
 // spmm.hip -- multi-RHS SpMM  Y = alpha*A*X + beta*Y  on the segmented engine's layout.
 //
 // X (n-by-k) and Y (m-by-k) are column-major with leading dimensions ldx, ldy.
-// One pass covers up to 8 columns (NC = the pass width rounded up to a power
-// of two; columns past the pass's kk repeat its last column and are never
-// stored), so A is read once per 8 columns instead of once per column.
-//
-// k_seg_multi walks the units of the single-column engine (seg.hip) with the
-// same chunks of 2048 nonzeros, 8 per thread, loaded once into registers:
-//  - a structural scan (closed flags and start-bit counts, DPP across the
-//    wave, wave totals through LDS) gives every piece its output index; it is
-//    the same for all columns;
-//  - per column, the thread gathers x_j at its 8 columns (straight from the
-//    column-major X: no transposed copy), forms its pieces and writes the
-//    ones it holds whole;
-//  - one segmented value scan over the pass's columns at once joins the
-//    pieces that cross threads, with the chunk's carry per column in
-//    registers;
-//  - the sums a chunk closes are gathered in LDS per column and stored in a
-//    coalesced pass (consecutive rows of one column of Y), or where they
-//    close if a chunk closes more than kSegOut of them.
+// A pass covers NC columns (NC = the pass's column count rounded up to a
+// power of two, at most kSpmmPass), so A is read once per pass:
+//  - k_xpose copies the pass's columns of X into XR, row-major with NC
+//    entries per column index (the extra entries zero), so the NC values one
+//    nonzero needs are one contiguous run (64 B at NC = 8);
+//  - k_seg_multi walks the units of the single-column engine (seg.hip).  The
+//    lanes of a wave form NC column groups of G = 64/NC lanes (lane = j*G + g):
+//    lane (j, g) handles column j of element group g, 8 consecutive nonzeros,
+//    so a chunk is 4*G groups = 2048/NC nonzeros.  The NC lanes of a group load
+//    the same columns and values (one broadcast), and each of its 8 gathers
+//    from XR touches G contiguous 8*NC-byte runs per wave;
+//  - the group splits its 8 products at its start bits into pieces exactly as
+//    the single-column engine does; a segmented scan over the G groups of
+//    each column (DPP within the column's lanes, wave totals through LDS)
+//    joins the pieces that cross groups, and the open sum of each column
+//    carries to the next chunk in its lanes;
+//  - the sums a chunk closes (at most 2048/NC + 1 per column) are gathered in
+//    LDS and stored in one coalesced pass: consecutive rows of one column of
+//    Y, or consecutive (slice, column) partials.
 // Long rows: one partial per (slice, column) at partial[slice * NC + j],
 // folded per column by k_long_reduce_multi.  Every addition happens in a
-// fixed order, so results are bitwise reproducible; a column's sums are
-// those of the single-column engine on that column.
+// fixed order, so results are bitwise reproducible.
 #include "seg_device.h"
 
 namespace spx {
+
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_xpose(const double *__restrict__ X, int64_t ldx, int n, int kk,
+                                                  double *__restrict__ XR) {
+  constexpr int C = 64;  // column indices of A (rows of X) per workgroup
+  __shared__ double s[C * (NC + 1)];
+  const int64_t c0 = (int64_t)blockIdx.x * C;
+  for (int t = threadIdx.x; t < C * NC; t += kBlock) {
+    const int j = t / C, c = t % C;  // consecutive threads read consecutive rows of one column
+    s[c * (NC + 1) + j] = (j < kk && c0 + c < n) ? X[c0 + c + (int64_t)j * ldx] : 0.0;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < C * NC; t += kBlock) {
+    const int c = t / NC, j = t % NC;  // consecutive threads write consecutive entries of XR
+    if (c0 + c < n) XR[(c0 + c) * NC + j] = s[c * (NC + 1) + j];
+  }
+}
 
 template <int NC>
 __device__ inline void multi_emit(const SpmmArgs &a, bool part, int j, int o, double s) {
@@ -37,172 +54,178 @@ __device__ inline void multi_emit(const SpmmArgs &a, bool part, int j, int o, do
   }
 }
 
+// One step of the per-column value scan: DPP row_shr / row_bcast within the
+// wave; a lane that is closed (really, or as its column's first group) takes
+// nothing from below, so no sum crosses from one column's lanes to the next.
+template <int CTRL, int ROWS, int BANKS>
+__device__ inline void col_scan_step(double &v, int &sg) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, BANKS, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, BANKS, false);
+  const int su = __builtin_amdgcn_update_dpp(0, sg, CTRL, ROWS, BANKS, false);
+  if (!sg) v += __hiloint2double(hi, lo);
+  sg |= su;
+}
+
+// Inclusive count scan across the wave (DPP), valid for lanes 0..G-1.
+template <int CTRL, int ROWS, int BANKS>
+__device__ inline void cnt_scan_step(int &cn) {
+  cn += __builtin_amdgcn_update_dpp(0, cn, CTRL, ROWS, BANKS, false);
+}
+
 template <int NC>
 __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
   constexpr int LE = 8;
-  constexpr int CH = kSegBlock * LE;
+  constexpr int G = kWave / NC;        // element groups per wave (lanes per column)
   constexpr int NW = kSegBlock / kWave;
-  __shared__ int s_ws[2][NW];  // per wave: closed flag, start-bit count (double-buffered per chunk)
+  constexpr int CH = NW * G * LE;      // nonzeros per chunk
+  constexpr int NOUT = CH + 8;         // outputs a chunk can close (CH + 1), padded
+  __shared__ int s_ws[2][NW];          // per wave: closed flag, start-bit count (double-buffered)
   __shared__ int s_wc[2][NW];
-  __shared__ double s_wv[2][NW][NC];  // per wave and column: scan value
-  __shared__ double s_out[NC][kSegOut];  // sums closed in the current chunk, by column and output
-  __shared__ uint8_t s_ok[kSegOut];      // which outputs are set
+  __shared__ double s_wv[2][NW][NC];   // per wave and column: scan value
+  __shared__ double s_out[NC][NOUT];   // the chunk's closed sums, by column and output
+  __shared__ int s_tag[NOUT];          // s_out[.][t] is this chunk's when s_tag[t] == tag
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-  for (int t = tid; t < kSegOut; t += kSegBlock) s_ok[t] = 0;
+  const int j = lane / G, g = lane % G;  // this lane's column and element group
+  for (int t = tid; t < NOUT; t += kSegBlock) s_tag[t] = 0;
   __syncthreads();
-  const double *xc[NC];
-#pragma unroll
-  for (int j = 0; j < NC; j++) xc[j] = a.X + (int64_t)(j < a.kk ? j : a.kk - 1) * a.ldx;
-  int par = 0;
+  const double *xr = a.XR + j;
+  const uint64_t below = ~0ull >> (63 - g);  // lanes g' <= g of a column, shifted to bit 0
+  int par = 0, tag = 0;
   for (int iter = 0;; iter++) {
     const int u = seg_unit(a.xr, iter);
     if (u < 0) break;
     const SegUnit U = a.units[u];
     const bool part = (U.flags & SEG_PARTIAL) != 0;
-    double carry[NC];  // per column: the sum still open at the end of the previous chunk
-#pragma unroll
-    for (int j = 0; j < NC; j++) carry[j] = 0.0;
-    int base = U.o0;  // outputs of the segments begun before the current chunk
+    double carry = 0.0;  // this lane's column: the sum still open at the end of the previous chunk
+    int base = U.o0;     // outputs of the segments begun before the current chunk
     for (int c = U.e0; c < U.e1; c += CH, par ^= 1) {
-      const int ea = c + tid * LE;
+      tag++;
+      const int ea = c + (wid * G + g) * LE;  // the group's first element
       const int nv = U.e1 - ea <= 0 ? 0 : (U.e1 - ea > LE ? LE : U.e1 - ea);
-      int32_t ci[LE];
-      double vv[LE];
+      double p[LE];
       uint32_t fb = 0;
-      int tclosed = 1;  // the thread's last piece ends with it
-      if (nv > 0) {  // groups of 8 are padded: a lane's 8 slots are always readable
-        const int B = c + wid * kSegWaveBlock;
-        const int L = seg_lanes(B, U.e1);
-        load8i(a.col, B, L, lane, ci);
-        Vec8<double>::load(a.val, B, L, lane, vv);
-        fb = __builtin_nontemporal_load(&a.start[ea >> 3]);
-        if (nv < LE) fb &= (1u << nv) - 1;
-        if (ea + nv < U.e1) tclosed = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]) & 1;
-      } else {
+      int tclosed = 1;  // the group's last piece ends with it
+      {
+        int32_t ci[LE];
+        double vv[LE];
+        if (nv > 0) {  // groups of 8 are padded: the 8 slots are always readable
+          const int off = ea - U.e0;
+          const int B = U.e0 + (off & ~(kSegWaveBlock - 1));  // its wave block and slot
+          const int vl = (off & (kSegWaveBlock - 1)) >> 3;
+          const int L = seg_lanes(B, U.e1);
+          if (U.flags & SEG_STAGE) {  // 16-bit offsets from cmin
+            load8u16(a.col16 + ea, ci);
+#pragma unroll
+            for (int i = 0; i < LE; i++) ci[i] += U.cmin;
+          } else {
+            load8i(a.col32 + U.cmin, B - U.e0, L, vl, ci);
+          }
+          Vec8<double>::load(a.val, B, L, vl, vv);
+          fb = __builtin_nontemporal_load(&a.start[ea >> 3]);
+          if (nv < LE) fb &= (1u << nv) - 1;
+          if (ea + nv < U.e1) tclosed = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]) & 1;
+        } else {
+#pragma unroll
+          for (int i = 0; i < LE; i++) {
+            ci[i] = 0;
+            vv[i] = 0.0;
+          }
+        }
 #pragma unroll
         for (int i = 0; i < LE; i++) {
-          ci[i] = 0;
-          vv[i] = 0.0;
+          const double t = vv[i] * xr[(int64_t)(i < nv ? ci[i] : 0) * NC];
+          p[i] = i < nv ? t : 0.0;
         }
       }
       const uint32_t inner = fb & ~1u;
       const int f1 = inner ? __builtin_ctz(inner) : nv;
       const int fl = inner ? 31 - __builtin_clz(inner) : 0;
+      double first_val = 0.0, last_val = 0.0;
+#pragma unroll
+      for (int i = 0; i < LE; i++) {
+        if (i < f1) first_val += p[i];
+        if (i >= fl && i < nv) last_val += p[i];
+      }
       const int has_close = nv > 0 && (inner != 0 || tclosed);
       const int cnt = __builtin_popcount(fb);
-
-      // structure: the output index of every piece
-      int sg = has_close, cn = cnt;
-      scan_step_sc<0x111, 0xf, 0xf>(sg, cn);  // row_shr:1
-      scan_step_sc<0x112, 0xf, 0xf>(sg, cn);  // row_shr:2
-      scan_step_sc<0x114, 0xf, 0xe>(sg, cn);  // row_shr:4
-      scan_step_sc<0x118, 0xf, 0xc>(sg, cn);  // row_shr:8
-      scan_step_sc<0x142, 0xa, 0xf>(sg, cn);  // row_bcast:15 into rows 1, 3
-      scan_step_sc<0x143, 0xc, 0xf>(sg, cn);  // row_bcast:31 into rows 2, 3
-      if (lane == kWave - 1) {
-        s_ws[par][wid] = sg;
+      // closed flags of this column's groups 0..g (every column has the same)
+      const uint64_t closed = __ballot(has_close);
+      const int sg_real = ((closed >> (j * G)) & below) != 0;
+      // values: segmented inclusive scan over the column's groups
+      double v = nv > 0 && !tclosed ? last_val : 0.0;
+      int sg = has_close | (g == 0);
+      col_scan_step<0x111, 0xf, 0xf>(v, sg);                        // row_shr:1
+      if constexpr (G > 2) col_scan_step<0x112, 0xf, 0xf>(v, sg);   // row_shr:2
+      if constexpr (G > 4) col_scan_step<0x114, 0xf, 0xe>(v, sg);   // row_shr:4
+      if constexpr (G > 8) col_scan_step<0x118, 0xf, 0xc>(v, sg);   // row_shr:8
+      if constexpr (G > 16) col_scan_step<0x142, 0xa, 0xf>(v, sg);  // row_bcast:15
+      if constexpr (G > 32) col_scan_step<0x143, 0xc, 0xf>(v, sg);  // row_bcast:31
+      // start-bit counts: inclusive scan over column 0's lanes, read per group
+      int cn = cnt;
+      cnt_scan_step<0x111, 0xf, 0xf>(cn);
+      if constexpr (G > 2) cnt_scan_step<0x112, 0xf, 0xf>(cn);
+      if constexpr (G > 4) cnt_scan_step<0x114, 0xf, 0xe>(cn);
+      if constexpr (G > 8) cnt_scan_step<0x118, 0xf, 0xc>(cn);
+      if constexpr (G > 16) cnt_scan_step<0x142, 0xa, 0xf>(cn);
+      if constexpr (G > 32) cnt_scan_step<0x143, 0xc, 0xf>(cn);
+      if constexpr (NC > 1) cn = __shfl(cn, g, kWave);
+      if (g == G - 1) s_wv[par][wid][j] = v;
+      if (lane == G - 1) {
+        s_ws[par][wid] = (closed & (~0ull >> (64 - G))) != 0;
         s_wc[par][wid] = cn;
       }
       __syncthreads();
+      double prev = carry, tot = carry;
       int cb = base, totc = base;
 #pragma unroll
       for (int w = 0; w < NW; w++) {
-        const int wc = s_wc[par][w];
-        if (w < wid) cb += wc;
+        const double wv = s_wv[par][w][j];
+        const int ws = s_ws[par][w], wc = s_wc[par][w];
+        if (w < wid) {
+          prev = wv + (ws ? 0.0 : prev);
+          cb += wc;
+        }
+        tot = wv + (ws ? 0.0 : tot);
         totc += wc;
       }
-      const int k0 = cb + cn - cnt + (int)(fb & 1) - 1;  // output of the thread's piece 0
-      const int kb = base - 1;  // the chunk's outputs are [kb, totc)
-      const bool via_lds = totc - kb <= kSegOut;  // uniform
-      auto put = [&](int j, int k, double val) {
-        if (via_lds) {
-          s_out[j][k - kb] = val;
-          s_ok[k - kb] = 1;
-        } else {
-          multi_emit<NC>(a, part, j, k, val);
-        }
+      const double c_incl = v + (sg_real ? 0.0 : prev);
+      double carry_in = __shfl_up(c_incl, 1, kWave);
+      if (g == 0) carry_in = prev;
+      const int kb = base - 1;  // the chunk's outputs are [kb, totc), at most CH + 1 of them
+      auto put = [&](int k, double val) {
+        s_out[j][k - kb] = val;
+        s_tag[k - kb] = tag;
       };
-
-      // per column: products, piece 0, the running piece, the whole pieces
-      double first[NC], v[NC];
-#pragma unroll
-      for (int j = 0; j < NC; j++) {
-        double p[LE];
-#pragma unroll
-        for (int i = 0; i < LE; i++) {
-          const double t = vv[i] * xc[j][i < nv ? ci[i] : 0];
-          p[i] = i < nv ? t : 0.0;
-        }
-        double fv = 0.0, lv = 0.0;
-#pragma unroll
-        for (int i = 0; i < LE; i++) {
-          if (i < f1) fv += p[i];
-          if (i >= fl && i < nv) lv += p[i];
-        }
-        first[j] = fv;
-        v[j] = nv > 0 && !tclosed ? lv : 0.0;
-        if (nv > 0) {
-          double acc = 0.0;
-          int open = 0, k = k0;
-#pragma unroll
-          for (int i = 1; i < LE; i++) {
-            if (i < nv) {
-              if ((inner >> i) & 1) {
-                if (open) put(j, k, acc);
-                k++;
-                acc = 0.0;
-                open = 1;
-              }
-              if (open) acc += p[i];
-            }
-          }
-          if (open && tclosed) put(j, k, acc);
-        }
-      }
-
-      // values: one segmented scan of the running pieces, all columns at once
-      int sv = has_close;
-      scan_step_v<0x111, 0xf, 0xf>(v, sv);
-      scan_step_v<0x112, 0xf, 0xf>(v, sv);
-      scan_step_v<0x114, 0xf, 0xe>(v, sv);
-      scan_step_v<0x118, 0xf, 0xc>(v, sv);
-      scan_step_v<0x142, 0xa, 0xf>(v, sv);
-      scan_step_v<0x143, 0xc, 0xf>(v, sv);
-      if (lane == kWave - 1) {
-#pragma unroll
-        for (int j = 0; j < NC; j++) s_wv[par][wid][j] = v[j];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < NC; j++) {
-        double prev = carry[j], tot = carry[j];
-#pragma unroll
-        for (int w = 0; w < NW; w++) {
-          const double wv = s_wv[par][w][j];
-          const int ws = s_ws[par][w];
-          if (w < wid) prev = wv + (ws ? 0.0 : prev);
-          tot = wv + (ws ? 0.0 : tot);
-        }
-        const double c_incl = v[j] + (sv ? 0.0 : prev);
-        double carry_in = __shfl_up(c_incl, 1, kWave);
-        if (lane == 0) carry_in = prev;
-        // piece 0 closes the segment open before this thread (or one that
+      if (nv > 0) {
+        const int k0 = cb + cn - cnt + (int)(fb & 1) - 1;  // output of the group's piece 0
+        // piece 0 closes the segment open before this group (or one that
         // begins at its first product)
-        if (has_close) put(j, k0, first[j] + carry_in);
-        carry[j] = tot;
-      }
-      if (via_lds) {
-        // s_out and s_ok are written again only after the next chunk's first
-        // barrier, which every thread reaches after this pass
-        __syncthreads();
-        for (int t = tid; t < totc - kb; t += kSegBlock) {
-          if (s_ok[t]) {
+        if (has_close) put(k0, first_val + carry_in);
+        double acc = 0.0;
+        int open = 0, k = k0;
 #pragma unroll
-            for (int j = 0; j < NC; j++) multi_emit<NC>(a, part, j, kb + t, s_out[j][t]);
-            s_ok[t] = 0;
+        for (int i = 1; i < LE; i++) {
+          if (i < nv) {
+            if ((inner >> i) & 1) {
+              if (open) put(k, acc);
+              k++;
+              acc = 0.0;
+              open = 1;
+            }
+            if (open) acc += p[i];
           }
         }
+        if (open && tclosed) put(k, acc);
       }
+      // s_out is written again only after the next chunk's first barrier,
+      // which every thread reaches after this pass
+      __syncthreads();
+      const int nout = totc - kb;
+      for (int q = tid; q < nout * NC; q += kSegBlock) {
+        const int jj = part ? q % NC : q / nout, t = part ? q / NC : q % nout;
+        if (s_tag[t] == tag) multi_emit<NC>(a, part, jj, kb + t, s_out[jj][t]);
+      }
+      carry = tot;
       base = totc;
     }
   }
@@ -270,6 +293,9 @@ static hipError_t spmm_pass(const SpmmArgs &a, hipStream_t st) {
     const int c = nb * cus / kXcd * kXcd;
     return c < kXcd ? kXcd : c;
   }();
+  if (a.n > 0)
+    hipLaunchKernelGGL((k_xpose<NC>), dim3((unsigned)((a.n + 63) / 64)), dim3(kBlock), 0, st, a.X, a.ldx, a.n, a.kk,
+                       a.XR);
   if (a.nempty > 0) {
     const int64_t t = a.nempty * a.kk;
     hipLaunchKernelGGL(k_empty_rows_multi, dim3((unsigned)((t + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a.empty,
@@ -299,6 +325,7 @@ hipError_t launch_spmm(const SpmmArgs &a, hipStream_t st) {
     case 2: return spmm_pass<2>(a, st);
     case 4: return spmm_pass<4>(a, st);
     case 8: return spmm_pass<8>(a, st);
+    case 16: return spmm_pass<16>(a, st);
     default: return hipErrorInvalidValue;
   }
 }

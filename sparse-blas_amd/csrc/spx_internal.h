@@ -68,9 +68,11 @@ constexpr int kSegBlock = 256;   // threads per segmented-engine workgroup (4 wa
 constexpr int kSegOut = 520;     // sums per chunk gathered in LDS before they are stored
 
 // One workgroup unit of the segmented engine: padded elements [e0, e1)
-// (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order, with
-// columns in [cmin, cmin + cwin).  Segment s0 + j writes output o0 + j: y row
-// o0 + j, or partial slot o0 + j in SEG_PARTIAL units.  Bit (e & 7) of
+// (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order.  A
+// SEG_STAGE unit's columns lie in [cmin, cmin + cwin) and are stored as
+// 16-bit offsets from cmin (col16); any other unit's columns are stored in
+// full in col32 from index cmin on.  Segment s0 + j writes output o0 + j: y
+// row o0 + j, or partial slot o0 + j in SEG_PARTIAL units.  Bit (e & 7) of
 // start[e >> 3] is set when element e begins a segment.
 struct SegUnit {
   int32_t e0, e1, s0, ns, o0, cmin, cwin, flags;
@@ -107,7 +109,8 @@ struct SegArgs {
   int64_t nunits;
   const int32_t *xr;     // XCD x owns units [xr[x], xr[x + 1]) (kXcd + 1 entries)
   const uint8_t *start;  // segment-start bits, one per padded element
-  const int32_t *col;
+  const uint16_t *col16;  // staged units: column - cmin, natural element order
+  const int32_t *col32;   // unstaged units: columns from cmin on, lane-fastest (seg_pos4 - e0)
   const V *val;
   const X *x;
   double alpha, beta;
@@ -117,18 +120,20 @@ struct SegArgs {
 
 // One multi-RHS pass (spmm.hip): columns of X and Y starting at the given
 // pointers, kk <= 32 of them.
-constexpr int kSpmmPass = 8;  // right-hand sides per fused SpMM pass (A read once per pass)
+constexpr int kSpmmPass = 16;  // right-hand sides per fused SpMM pass (A read once per pass)
 
 struct SpmmArgs {
   const SegUnit *units;
   int64_t nunits;
   const int32_t *xr;
   const uint8_t *start;  // segment-start bits, one byte per 8 elements
-  const int32_t *col;
+  const uint16_t *col16;  // as in SegArgs
+  const int32_t *col32;
   const double *val;
   const double *X;  // the pass's first column
   int64_t ldx;
-  int kk;  // columns in this pass, 1..kSpmmPass
+  int n, kk;   // columns of A; columns in this pass, 1..kSpmmPass
+  double *XR;  // n * spmm_width(kk) doubles: the pass's columns of X, row-major
   double alpha, beta;
   double *Y;  // the pass's first column
   int64_t ldy;

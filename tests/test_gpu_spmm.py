@@ -29,7 +29,7 @@ def run_multi(A, k, alpha, beta, pad=(0, 0), seed=1, vals=W.VAL_UNIFORM, **kw):
     return Y[:, :m], np.stack(refs), P
 
 
-@pytest.mark.parametrize("k", [1, 3, 8, 32, 37])
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 17, 32, 37])
 @pytest.mark.parametrize("kc", [dict(kernel=4, flags=spx.FLAG_FUSED_SPMM), dict(kernel=4), dict(kernel=2)],
                          ids=["k4-fused", "k4-loop", "k2-loop"])
 def test_spmm_matches_oracle(k, kc):
