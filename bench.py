@@ -154,8 +154,8 @@ def run_batch(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel", type=int, default=4)
     ap.add_argument("--parts", type=int, default=1)
     ap.add_argument("--items", type=int, default=4)
@@ -164,6 +164,8 @@ def main():
                     help="half-width of the regular rows' column band (0 = uniform columns)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=100)
+    ap.add_argument("--dropin-reps", type=int, default=3,
+                    help="calls of the host-pointer drop-in timed end to end (0 = skip)")
     ap.add_argument("--cold-reps", type=int, default=20,
                     help="executes timed with L2/MALL flushed before each (0 = skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -221,6 +223,21 @@ def main():
         _, cold_ms = plan.time(xd, yd, alpha, beta, warmup=1, reps=a.cold_reps, flush_bytes=512 << 20)
         log(f"cold caches: kernel {cold_ms:.4f} ms ({a.cold_reps} executes, 512 MiB overwritten before each)")
 
+    # the drop-in path end to end: host arrays in, host y out (upload, plan,
+    # SpMV, download, free per call), as the reference's callers use it
+    dropin_ms = None
+    if a.dropin_reps > 0:
+        yh = np.zeros(m)
+        ts = []
+        for _ in range(a.dropin_reps + 1):
+            t = time.perf_counter()
+            rc, _ = spx.spMV_mgpu_v1(m, n, nnz, alpha, A["val"], A["rowptr"], A["col"], x, beta, yh, 1, kernel=2)
+            ts.append(time.perf_counter() - t)
+            if rc != 0:
+                raise RuntimeError(f"spMV_mgpu_v1 returned {rc}")
+        dropin_ms = float(np.median(ts[1:])) * 1e3
+        log(f"drop-in spMV_mgpu_v1 (kernel 2) end to end: {dropin_ms:.1f} ms per call")
+
     if a.copy_mb > 0:
         nb = a.copy_mb << 20
         src, dst = spx.DeviceBuffer(nb), spx.DeviceBuffer(nb)
@@ -255,6 +272,9 @@ def main():
             "mode": "segmented units, one launch" if a.kernel == 4 else "nnz-balanced partitions (v1)",
         },
         "gflops": round(2 * nnz / (wall_ms * 1e-3) / 1e9, 2),
+        "dropin_end_to_end": None if dropin_ms is None else {
+            "entry": "spMV_mgpu_v1(kernel=2), host arrays", "ms_per_call": round(dropin_ms, 2),
+            "GBs": round(B / (dropin_ms * 1e-3) / 1e9, 2)},
         "ref_getB_gbps": round(W.ref_getB(m, nnz) / (wall_ms * 1e-3) / 1e9, 2),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

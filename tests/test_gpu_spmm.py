@@ -1,8 +1,9 @@
 """Multi-RHS SpMM (spx_plan_execute_multi): Y = alpha*A*X + beta*Y with X, Y
 column-major, against the CPU oracle column by column.  Kernel-4 plans with
-FLAG_FUSED_SPMM run the fused passes (up to 32 columns per read of A); the
-others loop the single-vector path.  Each fused output is one lane's sum in nonzero order, so
-small-integer inputs give the oracle's bits."""
+FLAG_FUSED_SPMM run the fused passes (up to 16 columns per read of A); the
+others loop the single-vector path.  Fused sums are formed in a fixed order
+(within a group of 8 nonzeros, then across groups), so small-integer inputs
+give the oracle's bits."""
 import numpy as np
 import pytest
 
