@@ -8,16 +8,17 @@
 //    nonzero needs are one contiguous run (64 B at NC = 8);
 //  - k_seg_multi walks the units of the single-column engine (seg.hip).  The
 //    lanes of a wave form NC column groups of G = 64/NC lanes (lane = j*G + g):
-//    lane (j, g) handles column j of element group g, 8 consecutive nonzeros,
-//    so a chunk is 4*G groups = 2048/NC nonzeros.  The NC lanes of a group load
-//    the same columns and values (one broadcast), and each of its 8 gathers
-//    from XR touches G contiguous 8*NC-byte runs per wave;
-//  - the group splits its 8 products at its start bits into pieces exactly as
-//    the single-column engine does; a segmented scan over the G groups of
+//    lane (j, g) handles column j of element group g, 16 consecutive nonzeros
+//    (two 8-element slots of the layout), so a chunk is 4*G groups =
+//    4096/NC nonzeros.  The NC lanes of a group load the same columns and
+//    values (one broadcast), and each of its 16 gathers from XR touches G
+//    contiguous 8*NC-byte runs per wave;
+//  - the group forms running sums of its products that restart at its start
+//    bits, exactly as the single-column engine does; a segmented scan over the G groups of
 //    each column (DPP within the column's lanes, wave totals through LDS)
 //    joins the pieces that cross groups, and the open sum of each column
 //    carries to the next chunk in its lanes;
-//  - the sums a chunk closes (at most 2048/NC + 1 per column) are gathered in
+//  - the sums a chunk closes (at most 4096/NC + 1 per column) are gathered in
 //    LDS and stored in one coalesced pass: consecutive rows of one column of
 //    Y, or consecutive (slice, column) partials.
 // Long rows: one partial per (slice, column) at partial[slice * NC + j],
@@ -74,35 +75,40 @@ __device__ inline void cnt_scan_step(int &cn) {
 
 template <int NC>
 __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
-  constexpr int LE = 8;
-  constexpr int G = kWave / NC;        // element groups per wave (lanes per column)
+  constexpr int LE = 8;                  // nonzeros per group per chunk
+  constexpr int CPL = NC < 4 ? NC : 4;   // columns per lane
+  constexpr int LPG = NC / CPL;          // lanes per element group
+  constexpr int G = kWave / LPG;         // element groups per wave
   constexpr int NW = kSegBlock / kWave;
-  constexpr int CH = NW * G * LE;      // nonzeros per chunk
-  constexpr int NOUT = CH + 8;         // outputs a chunk can close (CH + 1), padded
-  __shared__ int s_ws[2][NW];          // per wave: closed flag, start-bit count (double-buffered)
+  constexpr int CH = NW * G * LE;        // nonzeros per chunk
+  constexpr int NOUT = CH / 2 + 8;       // outputs gathered in LDS (more: stored where they close)
+  __shared__ int s_ws[2][NW];            // per wave: closed flag, start-bit count (double-buffered)
   __shared__ int s_wc[2][NW];
-  __shared__ double s_wv[2][NW][NC];   // per wave and column: scan value
-  __shared__ double s_out[NC][NOUT];   // the chunk's closed sums, by column and output
-  __shared__ int s_tag[NOUT];          // s_out[.][t] is this chunk's when s_tag[t] == tag
+  __shared__ double s_wv[2][NW][NC];     // per wave and column: scan value
+  __shared__ double s_out[NC][NOUT];     // the chunk's closed sums, by column and output
+  __shared__ int s_tag[NOUT];            // s_out[.][t] is this chunk's when s_tag[t] == tag
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-  const int j = lane / G, g = lane % G;  // this lane's column and element group
+  const int jl = lane / G, g = lane % G;  // this lane's column block and element group
+  const int j0 = jl * CPL;                // its first column
   for (int t = tid; t < NOUT; t += kSegBlock) s_tag[t] = 0;
   __syncthreads();
-  const double *xr = a.XR + j;
-  const uint64_t below = ~0ull >> (63 - g);  // lanes g' <= g of a column, shifted to bit 0
+  const double *xr = a.XR + j0;
+  const uint64_t below = ~0ull >> (63 - g);  // lanes g' <= g of a column block, shifted to bit 0
   int par = 0, tag = 0;
   for (int iter = 0;; iter++) {
     const int u = seg_unit(a.xr, iter);
     if (u < 0) break;
     const SegUnit U = a.units[u];
     const bool part = (U.flags & SEG_PARTIAL) != 0;
-    double carry = 0.0;  // this lane's column: the sum still open at the end of the previous chunk
-    int base = U.o0;     // outputs of the segments begun before the current chunk
+    double carry[CPL];  // per column: the sum still open at the end of the previous chunk
+#pragma unroll
+    for (int q = 0; q < CPL; q++) carry[q] = 0.0;
+    int base = U.o0;  // outputs of the segments begun before the current chunk
     for (int c = U.e0; c < U.e1; c += CH, par ^= 1) {
       tag++;
       const int ea = c + (wid * G + g) * LE;  // the group's first element
       const int nv = U.e1 - ea <= 0 ? 0 : (U.e1 - ea > LE ? LE : U.e1 - ea);
-      double p[LE];
+      double p[CPL][LE];
       uint32_t fb = 0;
       int tclosed = 1;  // the group's last piece ends with it
       {
@@ -132,35 +138,54 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
           }
         }
 #pragma unroll
-        for (int i = 0; i < LE; i++) {
-          const double t = vv[i] * xr[(int64_t)(i < nv ? ci[i] : 0) * NC];
-          p[i] = i < nv ? t : 0.0;
+        for (int i = 0; i < LE; i++) {  // the CPL entries of one XR row are contiguous
+          const double *row = xr + (int64_t)(i < nv ? ci[i] : 0) * NC;
+          double xq[CPL];
+          if constexpr (CPL == 4) {
+            const vd2 h0 = *reinterpret_cast<const vd2 *>(row), h1 = *reinterpret_cast<const vd2 *>(row + 2);
+            xq[0] = h0.x; xq[1] = h0.y; xq[2] = h1.x; xq[3] = h1.y;
+          } else if constexpr (CPL == 2) {
+            const vd2 h0 = *reinterpret_cast<const vd2 *>(row);
+            xq[0] = h0.x; xq[1] = h0.y;
+          } else {
+            xq[0] = row[0];
+          }
+#pragma unroll
+          for (int q = 0; q < CPL; q++) {
+            const double t = vv[i] * xq[q];
+            p[q][i] = i < nv ? t : 0.0;
+          }
         }
       }
+      // running sums that restart at every start bit (as in the one-column engine)
       const uint32_t inner = fb & ~1u;
       const int f1 = inner ? __builtin_ctz(inner) : nv;
-      const int fl = inner ? 31 - __builtin_clz(inner) : 0;
-      double first_val = 0.0, last_val = 0.0;
+      double v[CPL];
 #pragma unroll
-      for (int i = 0; i < LE; i++) {
-        if (i < f1) first_val += p[i];
-        if (i >= fl && i < nv) last_val += p[i];
+      for (int q = 0; q < CPL; q++) {
+#pragma unroll
+        for (int i = 1; i < LE; i++)
+          if (!((fb >> i) & 1)) p[q][i] = p[q][i - 1] + p[q][i];
+        double last = p[q][0];
+#pragma unroll
+        for (int i = 1; i < LE; i++)
+          if (i < nv) last = p[q][i];
+        v[q] = nv > 0 && !tclosed ? last : 0.0;
       }
       const int has_close = nv > 0 && (inner != 0 || tclosed);
       const int cnt = __builtin_popcount(fb);
-      // closed flags of this column's groups 0..g (every column has the same)
+      // closed flags of this column block's groups 0..g (every block has the same)
       const uint64_t closed = __ballot(has_close);
-      const int sg_real = ((closed >> (j * G)) & below) != 0;
-      // values: segmented inclusive scan over the column's groups
-      double v = nv > 0 && !tclosed ? last_val : 0.0;
+      const int sg_real = ((closed >> (jl * G)) & below) != 0;
+      // values: segmented inclusive scan over the block's groups, all CPL columns
       int sg = has_close | (g == 0);
-      col_scan_step<0x111, 0xf, 0xf>(v, sg);                        // row_shr:1
-      if constexpr (G > 2) col_scan_step<0x112, 0xf, 0xf>(v, sg);   // row_shr:2
-      if constexpr (G > 4) col_scan_step<0x114, 0xf, 0xe>(v, sg);   // row_shr:4
-      if constexpr (G > 8) col_scan_step<0x118, 0xf, 0xc>(v, sg);   // row_shr:8
-      if constexpr (G > 16) col_scan_step<0x142, 0xa, 0xf>(v, sg);  // row_bcast:15
-      if constexpr (G > 32) col_scan_step<0x143, 0xc, 0xf>(v, sg);  // row_bcast:31
-      // start-bit counts: inclusive scan over column 0's lanes, read per group
+      scan_step_v<0x111, 0xf, 0xf, CPL>(v, sg);                        // row_shr:1
+      if constexpr (G > 2) scan_step_v<0x112, 0xf, 0xf, CPL>(v, sg);   // row_shr:2
+      if constexpr (G > 4) scan_step_v<0x114, 0xf, 0xe, CPL>(v, sg);   // row_shr:4
+      if constexpr (G > 8) scan_step_v<0x118, 0xf, 0xc, CPL>(v, sg);   // row_shr:8
+      if constexpr (G > 16) scan_step_v<0x142, 0xa, 0xf, CPL>(v, sg);  // row_bcast:15
+      if constexpr (G > 32) scan_step_v<0x143, 0xc, 0xf, CPL>(v, sg);  // row_bcast:31
+      // start-bit counts: inclusive scan over block 0's lanes, read per group
       int cn = cnt;
       cnt_scan_step<0x111, 0xf, 0xf>(cn);
       if constexpr (G > 2) cnt_scan_step<0x112, 0xf, 0xf>(cn);
@@ -168,64 +193,74 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
       if constexpr (G > 8) cnt_scan_step<0x118, 0xf, 0xc>(cn);
       if constexpr (G > 16) cnt_scan_step<0x142, 0xa, 0xf>(cn);
       if constexpr (G > 32) cnt_scan_step<0x143, 0xc, 0xf>(cn);
-      if constexpr (NC > 1) cn = __shfl(cn, g, kWave);
-      if (g == G - 1) s_wv[par][wid][j] = v;
+      if constexpr (LPG > 1) cn = __shfl(cn, g, kWave);
+      if (g == G - 1) {
+#pragma unroll
+        for (int q = 0; q < CPL; q++) s_wv[par][wid][j0 + q] = v[q];
+      }
       if (lane == G - 1) {
         s_ws[par][wid] = (closed & (~0ull >> (64 - G))) != 0;
         s_wc[par][wid] = cn;
       }
       __syncthreads();
-      double prev = carry, tot = carry;
       int cb = base, totc = base;
 #pragma unroll
       for (int w = 0; w < NW; w++) {
-        const double wv = s_wv[par][w][j];
-        const int ws = s_ws[par][w], wc = s_wc[par][w];
-        if (w < wid) {
-          prev = wv + (ws ? 0.0 : prev);
-          cb += wc;
-        }
-        tot = wv + (ws ? 0.0 : tot);
+        const int wc = s_wc[par][w];
+        if (w < wid) cb += wc;
         totc += wc;
       }
-      const double c_incl = v + (sg_real ? 0.0 : prev);
-      double carry_in = __shfl_up(c_incl, 1, kWave);
-      if (g == 0) carry_in = prev;
-      const int kb = base - 1;  // the chunk's outputs are [kb, totc), at most CH + 1 of them
-      auto put = [&](int k, double val) {
-        s_out[j][k - kb] = val;
-        s_tag[k - kb] = tag;
-      };
-      if (nv > 0) {
-        const int k0 = cb + cn - cnt + (int)(fb & 1) - 1;  // output of the group's piece 0
-        // piece 0 closes the segment open before this group (or one that
-        // begins at its first product)
-        if (has_close) put(k0, first_val + carry_in);
-        double acc = 0.0;
-        int open = 0, k = k0;
+      const int kb = base - 1;  // the chunk's outputs are [kb, totc)
+      const bool via_lds = totc - kb <= NOUT;  // uniform
+      const uint32_t ends = nv > 0 ? ((fb >> 1) | ((uint32_t)tclosed << (nv - 1))) & ((1u << nv) - 1) : 0u;
 #pragma unroll
-        for (int i = 1; i < LE; i++) {
-          if (i < nv) {
-            if ((inner >> i) & 1) {
-              if (open) put(k, acc);
-              k++;
-              acc = 0.0;
-              open = 1;
+      for (int q = 0; q < CPL; q++) {
+        double prev = carry[q], tot = carry[q];
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          const double wv = s_wv[par][w][j0 + q];
+          const int ws = s_ws[par][w];
+          if (w < wid) prev = wv + (ws ? 0.0 : prev);
+          tot = wv + (ws ? 0.0 : tot);
+        }
+        const double c_incl = v[q] + (sg_real ? 0.0 : prev);
+        double carry_in = __shfl_up(c_incl, 1, kWave);
+        if (g == 0) carry_in = prev;
+        carry[q] = tot;
+        // every element that ends a segment emits its running sum (plus the
+        // carry from earlier groups for piece 0)
+        int k = cb + cn - cnt - 1;  // the output of the segment holding element i
+#pragma unroll
+        for (int i = 0; i < LE; i++) {
+          k += (fb >> i) & 1;
+          if ((ends >> i) & 1) {
+            const double s = i < f1 ? p[q][i] + carry_in : p[q][i];
+            if (via_lds) {
+              s_out[j0 + q][k - kb] = s;
+              s_tag[k - kb] = tag;
+            } else {
+              multi_emit<NC>(a, part, j0 + q, k, s);
             }
-            if (open) acc += p[i];
           }
         }
-        if (open && tclosed) put(k, acc);
       }
-      // s_out is written again only after the next chunk's first barrier,
-      // which every thread reaches after this pass
-      __syncthreads();
-      const int nout = totc - kb;
-      for (int q = tid; q < nout * NC; q += kSegBlock) {
-        const int jj = part ? q % NC : q / nout, t = part ? q / NC : q % nout;
-        if (s_tag[t] == tag) multi_emit<NC>(a, part, jj, kb + t, s_out[jj][t]);
+      if (via_lds) {
+        // s_out is written again only after the next chunk's first barrier,
+        // which every thread reaches after this pass
+        __syncthreads();
+        const int nout = totc - kb;
+        if (part) {  // partial[(kb + t) * NC + jj]: consecutive threads, consecutive slots
+          for (int q = tid; q < nout * NC; q += kSegBlock) {
+            const int t = q / NC, jj = q % NC;
+            if (s_tag[t] == tag) multi_emit<NC>(a, true, jj, kb + t, s_out[jj][t]);
+          }
+        } else {  // column jj of Y: kSegBlock / NC threads per column, consecutive rows
+          constexpr int TPC = kSegBlock / NC;
+          const int jj = tid / TPC;
+          for (int t = tid % TPC; t < nout; t += TPC)
+            if (s_tag[t] == tag) multi_emit<NC>(a, false, jj, kb + t, s_out[jj][t]);
+        }
       }
-      carry = tot;
       base = totc;
     }
   }
