@@ -109,7 +109,7 @@ struct spx_plan {
   // segmented engine: the long rows' (column block, row) slices for every
   // kernel, plus the regular rows for kernel 4
   SegUnit *d_sunits = nullptr;
-  int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work
+  int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work: all, staged, unstaged (3 x 9)
   uint8_t *d_sstart = nullptr;
   double *d_xr = nullptr;     // multi-RHS scratch: the pass's X columns, row-major
   double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials
@@ -120,6 +120,8 @@ struct spx_plan {
   double *d_sval = nullptr;
   float *d_sval32 = nullptr;
   int64_t nsunits = 0, nslong = 0;  // units; the first nslong sweep long rows
+  int64_t nsstaged = 0;              // split plans: the first nsstaged units stage x
+  bool seg_split = false;            // staged and unstaged units run as two concurrent kernels
   int64_t nlseg = 0;                 // long-row slices
   bool sstage = false;
   hipEvent_t lfork = nullptr, ldone = nullptr;
@@ -378,20 +380,56 @@ static int seg_upload(spx_plan *P, SegHost &H) {
     for (int s = 0; ok && s < U.ns; s++) ok = H.out[U.s0 + s] == U.o0 + s;  // consecutive outputs
     if (!ok) return fail(SPX_EDEV, "segmented engine: inconsistent unit table");
   }
-  P->nsunits = (int64_t)H.units.size();
-  P->sstage = H.stage;
-  // XCD ranges of about equal work: nonzeros, staged x entries, and a fixed
-  // cost per unit
-  std::vector<double> wsum(H.units.size() + 1, 0.0);
-  for (size_t u = 0; u < H.units.size(); u++) {
-    const SegUnit &U = H.units[u];
-    wsum[u + 1] = wsum[u] + (U.e1 - U.e0) + ((U.flags & SEG_STAGE) ? 0.5 * U.cwin : 0.0) + 256.0;
+  // Staged and unstaged units run as two concurrent kernels (staged units
+  // ordered first, each class with its own XCD ranges) when the unstaged
+  // ones carry a real share of the work and their gathers range over more
+  // x than one XCD's L2 holds (4 MiB): those miss L2 anyway and gain from
+  // the extra workgroups the kernel without the LDS window fits per CU.
+  // Gathers that stay within L2 do better with fewer working sets competing
+  // for it, and a few unstaged units ride along in the staged kernel, in
+  // their place in the unit order.
+  auto work = [](const SegUnit &U) {  // nonzeros, staged x entries, a fixed cost per unit
+    return (double)(U.e1 - U.e0) + ((U.flags & SEG_STAGE) ? 0.5 * U.cwin : 0.0) + 256.0;
+  };
+  {
+    double wall = 0.0, wu = 0.0, span = 0.0, el = 0.0;  // el: elements of unstaged units
+    int64_t nst = 0;
+    for (const SegUnit &U : H.units) {
+      wall += work(U);
+      if (U.flags & SEG_STAGE) {
+        nst++;
+      } else {
+        wu += work(U);
+        span += (double)(U.e1 - U.e0) * U.cwin;
+        el += U.e1 - U.e0;
+      }
+    }
+    const double xbytes = 8.0;  // an x entry (FP64, or the FP32 hi/lo pair)
+    P->seg_split = nst > 0 && wu >= 0.1 * wall && el > 0 && span / el * xbytes > 4.0 * (1 << 20);
+    if (P->seg_split)
+      std::stable_partition(H.units.begin(), H.units.end(),
+                            [](const SegUnit &U) { return (U.flags & SEG_STAGE) != 0; });
+    P->nsunits = (int64_t)H.units.size();
+    P->nsstaged = P->seg_split ? nst : 0;
+    P->sstage = nst > 0;
   }
-  std::vector<int32_t> xr(kXcd + 1, 0);
-  for (int x = 1; x < kXcd; x++)
-    xr[x] = (int32_t)(std::lower_bound(wsum.begin(), wsum.end(), wsum.back() * x / kXcd) - wsum.begin());
-  xr[kXcd] = (int32_t)H.units.size();
-  for (int x = 1; x <= kXcd; x++) xr[x] = std::max(xr[x], xr[x - 1]);
+  // XCD ranges of about equal work over all units, and (split plans) over
+  // the staged ones and over the rest
+  std::vector<double> wsum(H.units.size() + 1, 0.0);
+  for (size_t u = 0; u < H.units.size(); u++) wsum[u + 1] = wsum[u] + work(H.units[u]);
+  std::vector<int32_t> xr(3 * (kXcd + 1), 0);
+  auto ranges = [&](int64_t lo, int64_t hi, int32_t *r) {
+    r[0] = (int32_t)lo;
+    for (int x = 1; x < kXcd; x++) {
+      const double w = wsum[lo] + (wsum[hi] - wsum[lo]) * x / kXcd;
+      r[x] = (int32_t)(std::lower_bound(wsum.begin() + lo, wsum.begin() + hi, w) - wsum.begin());
+    }
+    r[kXcd] = (int32_t)hi;
+    for (int x = 1; x <= kXcd; x++) r[x] = std::max(r[x], r[x - 1]);
+  };
+  ranges(0, P->nsunits, xr.data());
+  ranges(0, P->nsstaged, xr.data() + (kXcd + 1));
+  ranges(P->nsstaged, P->nsunits, xr.data() + 2 * (kXcd + 1));
   // the device order of the columns and values: values lane-fastest per wave
   // block; columns of staged units as 16-bit window offsets in element
   // order (a lane's 8 are one 16-byte load), those of other units in full,
@@ -428,7 +466,7 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   std::vector<int32_t>().swap(H.col);
   int rc;
   if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
-  if ((rc = P->upload(&P->d_sxr, xr.data(), kXcd + 1))) return rc;
+  if ((rc = P->upload(&P->d_sxr, xr.data(), 3 * (kXcd + 1)))) return rc;
   if ((rc = P->upload(&P->d_sstart, H.start.data(), (int64_t)H.start.size()))) return rc;
   if ((rc = P->upload(&P->d_scol16, c16.data(), total))) return rc;
   if (!c32.empty() && (rc = P->upload(&P->d_scol32, c32.data(), (int64_t)c32.size()))) return rc;
@@ -613,6 +651,11 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
         [&](int64_t i) { return rows[i] != rows[i - 1] + 1; }, src_col, src_val, seg_target, true, 0);
     int rc;
     if ((rc = seg_upload(P, H))) return rc;
+    if (P->seg_split) {
+      HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&P->ldone, hipEventDisableTiming));
+    }
     P->nempty = (int64_t)empty.size();
     if ((rc = P->upload(&P->d_empty, empty.data(), P->nempty))) return rc;
     if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
@@ -947,15 +990,23 @@ static KArgs<V, X> make_args(spx_plan *P, const V *val, const X *x, double alpha
   return a;
 }
 
-static hipError_t launch_seg_all(spx_plan *P, const double *x, double alpha, double beta, double *y, hipStream_t st) {
+// The segmented engine over one class of units: SEG_ALL (one kernel; the
+// one with the LDS window if any unit stages), SEG_STAGED or SEG_UNSTAGED.
+enum { SEG_ALL = 0, SEG_STAGED = 1, SEG_UNSTAGED = 2 };
+static hipError_t launch_seg(spx_plan *P, int cls, const double *x, double alpha, double beta, double *y,
+                             hipStream_t st) {
+  const int64_t cnt = cls == SEG_ALL ? P->nsunits : cls == SEG_STAGED ? P->nsstaged : P->nsunits - P->nsstaged;
+  if (cnt <= 0) return hipSuccess;
+  const int32_t *xr = P->d_sxr + cls * (kXcd + 1);
+  const bool stage = cls == SEG_STAGED || (cls == SEG_ALL && P->sstage);
   if (P->precision == SPX_PREC_F32) {
-    SegArgs<float, XPair> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval32, P->d_xhl,
-                               alpha, beta, y, P->d_partial};
-    return launch_seg_f32(a, P->sstage, st);
+    SegArgs<float, XPair> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval32,
+                               P->d_xhl, alpha, beta, y, P->d_partial};
+    return launch_seg_f32(a, stage, st);
   }
-  SegArgs<double, double> a = {P->d_sunits, P->nsunits, P->d_sxr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval, x,
-                               alpha, beta, y, P->d_partial};
-  return launch_seg_f64(a, P->sstage, st);
+  SegArgs<double, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval,
+                               x, alpha, beta, y, P->d_partial};
+  return launch_seg_f64(a, stage, st);
 }
 
 // Enqueue one SpMV.  t0/t1 (optional) bracket the SpMV kernels proper (the
@@ -973,7 +1024,16 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
     // rows of A without nonzeros; the long rows are written by the reduction
     if (P->nempty > 0) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
     if (t0) HIP_TRY(hipEventRecord(t0, st));
-    HIP_TRY(launch_seg_all(P, x, alpha, beta, y, st));
+    if (P->seg_split) {  // the unstaged units on the second stream, concurrently
+      HIP_TRY(hipEventRecord(P->lfork, st));
+      HIP_TRY(hipStreamWaitEvent(P->lstream, P->lfork, 0));
+      HIP_TRY(launch_seg(P, SEG_STAGED, x, alpha, beta, y, st));
+      HIP_TRY(launch_seg(P, SEG_UNSTAGED, x, alpha, beta, y, P->lstream));
+      HIP_TRY(hipEventRecord(P->ldone, P->lstream));
+      HIP_TRY(hipStreamWaitEvent(st, P->ldone, 0));
+    } else {
+      HIP_TRY(launch_seg(P, SEG_ALL, x, alpha, beta, y, st));
+    }
     if (t1) HIP_TRY(hipEventRecord(t1, st));
     if (P->nlong > 0)
       HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->d_lsp, P->d_lseg, P->d_lrows, alpha, beta, true, y, st));
@@ -984,7 +1044,7 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
   if (longpath) {
     HIP_TRY(hipEventRecord(P->lfork, st));
     HIP_TRY(hipStreamWaitEvent(P->lstream, P->lfork, 0));
-    HIP_TRY(launch_seg_all(P, x, alpha, beta, y, P->lstream));
+    HIP_TRY(launch_seg(P, SEG_ALL, x, alpha, beta, y, P->lstream));
     HIP_TRY(hipEventRecord(P->ldone, P->lstream));
   }
   if (P->kernel == SPX_KERNEL_TILE) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
