@@ -164,6 +164,8 @@ def main():
                     help="half-width of the regular rows' column band (0 = uniform columns)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=100)
+    ap.add_argument("--uniform-reps", type=int, default=20,
+                    help="executes timed on the uniform-column variant of the matrix (0 = skip)")
     ap.add_argument("--dropin-reps", type=int, default=3,
                     help="calls of the host-pointer drop-in timed end to end (0 = skip)")
     ap.add_argument("--cold-reps", type=int, default=20,
@@ -223,6 +225,22 @@ def main():
         _, cold_ms = plan.time(xd, yd, alpha, beta, warmup=1, reps=a.cold_reps, flush_bytes=512 << 20)
         log(f"cold caches: kernel {cold_ms:.4f} ms ({a.cold_reps} executes, 512 MiB overwritten before each)")
 
+    # the same sizes with every row's columns uniform at random (no locality
+    # at all: the gathers, not the matrix stream, bound it)
+    uniform = None
+    if a.uniform_reps > 0:
+        Au = W.circuit(window=0)
+        Pu = spx.Plan(Au["rowptr"], Au["col"], Au["val"], n=n, kernel=a.kernel, parts=a.parts, items=a.items,
+                      sigma=a.sigma)
+        tu, ku, wu = Pu.time(xd, yd, alpha, beta, warmup=3, reps=a.uniform_reps, wall=True)
+        Bu = W.bytes_model(Au["m"], Au["n"], Au["nnz"], 8, beta != 0.0)
+        uniform = {"workload": f"same generator, every row's columns uniform at random (nnz={Au['nnz']})",
+                   "ms_per_step": round(wu, 5), "kernel_ms": round(ku, 5), "GBs": round(Bu / (wu * 1e-3) / 1e9, 2),
+                   "kernel_frac_of_hbm_peak": round(Bu / (ku * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"uniform columns: wall {wu:.4f} ms/step, kernel {ku:.4f} ms")
+        Pu.close()
+        del Au
+
     # the drop-in path end to end: host arrays in, host y out (upload, plan,
     # SpMV, download, free per call), as the reference's callers use it
     dropin_ms = None
@@ -272,6 +290,7 @@ def main():
             "mode": "segmented units, one launch" if a.kernel == 4 else "nnz-balanced partitions (v1)",
         },
         "gflops": round(2 * nnz / (wall_ms * 1e-3) / 1e9, 2),
+        "uniform_columns": uniform,
         "dropin_end_to_end": None if dropin_ms is None else {
             "entry": "spMV_mgpu_v1(kernel=2), host arrays", "ms_per_call": round(dropin_ms, 2),
             "GBs": round(B / (dropin_ms * 1e-3) / 1e9, 2)},
