@@ -54,7 +54,8 @@ def run_batch(a):
     U[-1,1)): the plain form (one FP64 plan per matrix, the drop-in default
     schedule: kernel 2, 8 items, one partition, one execute per matrix)
     against the learned-selector + FP32-storage batched form (all matrices
-    in one plan, schedule from the selector, one hipGraph replay per batch).
+    in one plan, one hipGraph replay per batch; the schedule is the fastest of
+    the selector's shortlist timed on a sample of the batch).
     Prints one JSON line: matrices/s of both, the ratio, and the FP32 form's
     normwise error against the oracle on sampled members."""
     import ctypes as C
@@ -104,17 +105,35 @@ def run_batch(a):
     for i, A in enumerate(mats):
         rp[off[i] + 1:off[i + 1] + 1] = A["rowptr"][1:] + nz_off[i]
     col = np.concatenate([A["col"] + np.int32(off[i]) for i, A in enumerate(mats)])
-    sched, kw = S.select(dict(rowptr=rp, col=col, m=M, n=M))
+    cands = S.shortlist(S.features(dict(rowptr=rp, col=col, m=M, n=M)), top=a.autotune_top)
     del col, rp
-    log(f"selector: schedule {sched} {kw} ({time.time() - t0:.1f}s)")
+    log(f"selector: shortlist {cands} ({time.time() - t0:.1f}s)")
+
+    def batch_kw(kw):
+        bkw = dict(kw)
+        if "parts" in bkw:  # kernels 1-3: the selector's partition count over nnz-balanced streams
+            bkw["mode"] = spx.MODE_NNZ
+        return bkw
+
+    # autotuner: the shortlist timed on every 16th member, packed in FP32 as
+    # the real batch will be
+    t0 = time.time()
+    sample = mats[::16]
+    ms_ = sum(A["m"] for A in sample)
+    xs, ys = spx.DeviceBuffer.from_array(x[:ms_]), spx.DeviceBuffer(ms_ * 8)
+    sched, ttimes = S.autotune(lambda kw: spx.BatchPlan(sample, precision=spx.PREC_F32, **batch_kw(kw)),
+                               xs, ys, cands)
+    xs.free()
+    ys.free()
+    kw = S.schedule(sched)
+    log(f"autotune on {len(sample)} members: " +
+        ", ".join(f"{s} {S.schedule(s)} {t:.3f} ms" for s, t in ttimes.items()) +
+        f" -> {sched} ({time.time() - t0:.1f}s)")
 
     out = {}
     for prec, tag in ((spx.PREC_F32, "fp32"), (spx.PREC_F64, "fp64")):
         t0 = time.time()
-        bkw = dict(kw)
-        if "parts" in bkw:  # kernels 1-3: the selector's partition count over nnz-balanced streams
-            bkw["mode"] = spx.MODE_NNZ
-        bp = spx.BatchPlan(mats, precision=prec, **bkw)
+        bp = spx.BatchPlan(mats, precision=prec, **batch_kw(kw))
         g = spx.Graph(bp, xd, yd)
         ms = g.time(warmup=a.warmup, reps=a.steps)
         log(f"batched {tag}: {ms:.3f} ms per batch (plan {time.time() - t0:.1f}s)")
@@ -141,6 +160,8 @@ def run_batch(a):
         "vs_baseline": None, "dtype": "f32 storage, f64 accumulation", "data": "synthetic",
         "config": {"workload": f"{len(mats)} CSR matrices, n lognormal(ln 14142, 1.32) in [1e3, 2e5], "
                                f"Zipf(1.3) rows on [1, 2048], {NNZ} nnz total", "schedule": kw},
+        "autotune": {"shortlist": [int(c) for c in cands], "chosen": int(sched),
+                     "sample_ms": {str(k): round(v, 4) for k, v in ttimes.items()}},
         "plain_fp64_matrices_per_s": round(len(mats) / (plain_ms * 1e-3), 1),
         "batched_fp64_matrices_per_s": round(len(mats) / (ms64 * 1e-3), 1),
         "speedup_vs_plain": round(plain_ms / ms32, 3),
@@ -176,6 +197,8 @@ def main():
     ap.add_argument("--workload", choices=["circuit", "batch"], default="circuit",
                     help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--autotune-top", type=int, default=4,
+                    help="batch: schedules of the selector's shortlist the autotuner times")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build drives one MI355X; --gpus must be 1 (the multi-GPU "
@@ -232,7 +255,9 @@ def main():
         Au = W.circuit(window=0)
         Pu = spx.Plan(Au["rowptr"], Au["col"], Au["val"], n=n, kernel=a.kernel, parts=a.parts, items=a.items,
                       sigma=a.sigma)
-        tu, ku, wu = Pu.time(xd, yd, alpha, beta, warmup=3, reps=a.uniform_reps, wall=True)
+        yu = spx.DeviceBuffer(m * 8)  # its own y: the headline's y is checked below
+        tu, ku, wu = Pu.time(xd, yu, alpha, beta, warmup=3, reps=a.uniform_reps, wall=True)
+        yu.free()
         Bu = W.bytes_model(Au["m"], Au["n"], Au["nnz"], 8, beta != 0.0)
         uniform = {"workload": f"same generator, every row's columns uniform at random (nnz={Au['nnz']})",
                    "ms_per_step": round(wu, 5), "kernel_ms": round(ku, 5), "GBs": round(Bu / (wu * 1e-3) / 1e9, 2),
@@ -337,6 +362,8 @@ def main():
         }
         err = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
         out["err_vs_oracle"] = err
+        if not err <= 1e-13:
+            raise AssertionError(f"SpMV differs from the oracle: normwise relative error {err:.3e} > 1e-13")
         log(f"cpu baseline {t_cpu * 1e3:.1f} ms/SpMV (1 thread), {t_omp * 1e3:.1f} ms/SpMV ({threads} threads); "
             f"normwise rel err vs oracle {err:.3e}")
     print(json.dumps(out), flush=True)

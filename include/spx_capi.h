@@ -116,7 +116,9 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
  * [8] items [9] units (workgroup work items) [10] carries [11] tiles
  * [12] device bytes held [13] precision [14] queues [15] long rows
  * [16] long-row nonzeros [17] column blocks [18] carry runs
- * [19] long-row threshold. Returns the count. */
+ * [19] long-row threshold [20] segmented units that stage their x window
+ * [21] 1 when staged and unstaged units run as two concurrent kernels.
+ * Returns the count. */
 int spx_plan_info(spx_plan_t p, int64_t *out, int cap);
 
 /* Copy plan internals to host for bit-exact checks.  `what`:

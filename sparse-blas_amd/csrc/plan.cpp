@@ -122,6 +122,7 @@ struct spx_plan {
   int64_t nsunits = 0, nslong = 0;  // units; the first nslong sweep long rows
   int64_t nsstaged = 0;              // split plans: the first nsstaged units stage x
   bool seg_split = false;            // staged and unstaged units run as two concurrent kernels
+  int64_t nstaged_units = 0;         // units that stage their x window (any order)
   int64_t nlseg = 0;                 // long-row slices
   bool sstage = false;
   hipEvent_t lfork = nullptr, ldone = nullptr;
@@ -411,6 +412,7 @@ static int seg_upload(spx_plan *P, SegHost &H) {
                             [](const SegUnit &U) { return (U.flags & SEG_STAGE) != 0; });
     P->nsunits = (int64_t)H.units.size();
     P->nsstaged = P->seg_split ? nst : 0;
+    P->nstaged_units = nst;
     P->sstage = nst > 0;
   }
   // XCD ranges of about equal work over all units, and (split plans) over
@@ -1142,7 +1144,7 @@ int spx_plan_info(spx_plan_t p, int64_t *out, int cap) {
                  p->sigma,  p->lanes,    p->items,
                  p->kernel == SPX_KERNEL_SEG ? p->nsunits : p->nunits, p->ncarry, p->ntiles,
                  p->bytes,  p->precision, p->queues,   p->nlong,  p->long_nnz, p->nblocks,
-                 p->nruns,  p->long_th};
+                 p->nruns,  p->long_th, p->nstaged_units, p->seg_split ? 1 : 0};
   int c = std::min<int>(cap, (int)(sizeof(v) / sizeof(v[0])));
   for (int i = 0; i < c; i++) out[i] = v[i];
   return c;

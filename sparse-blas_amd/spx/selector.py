@@ -28,7 +28,7 @@ import time
 
 import numpy as np
 
-from . import DeviceBuffer, Plan, check, lib, synchronize
+from . import DeviceBuffer, Plan, SpxError, check, lib, synchronize
 from . import workloads as W
 
 NSCHED = 96
@@ -194,6 +194,40 @@ def select(A, model=None):
     model = model or MLP.load()
     s = int(model.choose(features(A)[None, :])[0])
     return s, schedule(s)
+
+
+def shortlist(f, top=4, model=None):
+    """The model's `top` highest-scoring distinct schedules for feature row f,
+    best first (the first is what select() picks)."""
+    model = model or MLP.load()
+    z = model.forward(np.asarray(f, np.float32)[None, :])[0]
+    out = []
+    for s in np.argsort(-z, kind="stable"):
+        r = representative(int(s))
+        if r not in out:
+            out.append(r)
+        if len(out) == top:
+            break
+    return out
+
+
+def autotune(make_plan, x_dev, y_dev, candidates, warmup=2, reps=5):
+    """Time each candidate schedule on the plans make_plan(kwargs) builds (the
+    real operand: a batch sample in its final precision, say) and return
+    (fastest schedule, {schedule: ms per execute}).  A candidate whose plan
+    cannot be built is skipped."""
+    times = {}
+    for s in candidates:
+        try:
+            P = make_plan(schedule(s))
+        except SpxError as e:
+            print(f"autotune: schedule {s} {schedule(s)}: {e}", file=sys.stderr, flush=True)
+            continue
+        times[s] = P.time(x_dev, y_dev, warmup=warmup, reps=reps)[0]
+        P.close()
+    if not times:
+        raise SpxError(-1, "autotune: no candidate schedule could be built")
+    return min(times, key=times.get), times
 
 
 def exhaustive(A, x=None, warmup=2, reps=5):

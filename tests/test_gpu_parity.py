@@ -280,6 +280,34 @@ def c2():
     return A, x, ref
 
 
+@pytest.mark.parametrize("precision", [spx.PREC_F64, spx.PREC_F32])
+@pytest.mark.parametrize("items", [1, 4])
+def test_kernel4_split_into_two_concurrent_kernels(items, precision):
+    """A circuit-like matrix whose regular rows have uniform columns over
+    600,000: their units cannot stage x and gather over 4.8 MB, so kernel 4
+    runs them as a second kernel beside the staged long-row units."""
+    A = W.circuit(m=600_000, total=4_000_000, n_dense=20, window=0, seed=5)
+    x = W.vector(A["n"], 23)
+    y0 = W.vector(A["m"], 24)
+    print(f"split case items={items} precision={precision} nnz={A['nnz']}", flush=True)
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=4, items=items, precision=precision)
+    info = P.info()
+    assert info["split"] == 1 and 0 < info["staged_units"] < info["units"], info
+    for alpha, beta in [(1.0, 0.0), (0.5, -2.0)]:
+        y = P(x, alpha, beta, y0)
+        if precision == spx.PREC_F64:
+            check_fp64(A, x, y, alpha, beta, y0 if beta != 0 else None)
+        else:  # FP32 values exactly, x as a (hi, lo) pair: within 1e-12 of the FP32-value loop
+            ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, alpha, beta,
+                                     y0 if beta != 0 else None)
+            assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    # the same plan twice gives the same bits (no order depends on the two kernels' timing)
+    assert np.array_equal(P(x, 0.5, -2.0, y0), P(x, 0.5, -2.0, y0))
+    # and a banded matrix of the same size does not split
+    Ab = W.circuit(m=600_000, total=4_000_000, n_dense=20, window=1024, seed=5)
+    assert spx.Plan(Ab["rowptr"], Ab["col"], Ab["val"], n=Ab["n"], kernel=4, items=items).info()["split"] == 0
+
+
 @pytest.mark.parametrize("kc", [dict(kernel=2), dict(kernel=3), dict(kernel=2, parts=8),
                                 dict(kernel=2, mode=spx.MODE_QUEUE, nb=59_500_000 // 8, queues=8),
                                 dict(kernel=4), dict(kernel=4, items=8)])

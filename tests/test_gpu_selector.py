@@ -7,6 +7,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle
 from oracle import mlp
 import spx
 from spx import selector as S
@@ -102,6 +103,25 @@ def test_selected_plan_is_correct():
     y = P(x)
     ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
     assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-13
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained selector model")
+def test_shortlist_and_autotune():
+    """The shortlist starts with the selector's pick and holds distinct
+    schedules; the autotuner returns the fastest of the ones it timed, and
+    the schedule it returns computes the oracle's result."""
+    A, _ = S.training_matrix(10_005)
+    f = S.features(A)
+    sl = S.shortlist(f, top=4)
+    assert len(sl) == 4 and len(set(sl)) == 4
+    assert sl[0] == S.representative(S.select(A)[0])
+    x = W.vector(A["n"], 3)
+    xd, yd = spx.DeviceBuffer.from_array(x), spx.DeviceBuffer(A["m"] * 8)
+    best, t = S.autotune(lambda kw: spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], **kw), xd, yd, sl)
+    assert set(t) <= set(sl) and best in t and t[best] == min(t.values()) and t[best] > 0
+    y = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], **S.schedule(best))(x)
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x)
+    assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
 
 
 @pytest.mark.skipif(not (os.path.exists(MODEL) and os.path.exists(HELDOUT) and os.path.exists(TRAIN)),
