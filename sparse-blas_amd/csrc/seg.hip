@@ -37,7 +37,7 @@ __device__ inline void seg_emit(const SegArgs<V, X> &a, bool part, int o, double
 }
 
 template <typename V, typename X, bool STAGE>
-__global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
+__global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_seg(SegArgs<V, X> a) {
   constexpr int LE = 8;
   constexpr int CH = kSegBlock * LE;
   constexpr int NW = kSegBlock / kWave;
@@ -58,17 +58,33 @@ __global__ __launch_bounds__(kSegBlock) void k_seg(SegArgs<V, X> a) {
     const bool stg = STAGE && (U.flags & SEG_STAGE);
     const bool part = (U.flags & SEG_PARTIAL) != 0;
     if (stg) {
-      __syncthreads();  // the previous unit's gathers from s_x are done
-      for (int k0 = 0; k0 < U.cwin; k0 += 4 * kSegBlock) {
-        X xv[4];
+      // The window is loaded in two rounds of NX entries per thread, each
+      // round's loads issued together (two memory latencies per unit, and
+      // few enough registers to keep the kernel's occupancy).  The first
+      // round's loads are in flight while the previous unit's gathers from
+      // s_x finish.
+      constexpr int NX = STAGE ? (kSegStage + 2 * kSegBlock - 1) / (2 * kSegBlock) : 1;
+      X xv[NX];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int k = k0 + tid + i * kSegBlock;
+      for (int i = 0; i < NX; i++) {
+        const int k = tid + i * kSegBlock;
+        if (k < U.cwin) xv[i] = a.x[U.cmin + k];
+      }
+      __syncthreads();  // the previous unit's gathers from s_x are done
+#pragma unroll
+      for (int i = 0; i < NX; i++) {
+        const int k = tid + i * kSegBlock;
+        if (k < U.cwin) s_x[k] = xv[i];
+      }
+      if (U.cwin > NX * kSegBlock) {
+#pragma unroll
+        for (int i = 0; i < NX; i++) {
+          const int k = tid + (NX + i) * kSegBlock;
           if (k < U.cwin) xv[i] = a.x[U.cmin + k];
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int k = k0 + tid + i * kSegBlock;
+        for (int i = 0; i < NX; i++) {
+          const int k = tid + (NX + i) * kSegBlock;
           if (k < U.cwin) s_x[k] = xv[i];
         }
       }
