@@ -48,6 +48,95 @@ def profiled_traffic(kernel, items, window):
     return None
 
 
+def run_precision(a):
+    """The precision form on the rail-like (4,284 x 1,092,610, values 1.0) or
+    ASIC-like (682,862 square, lognormal values with random sign) matrix:
+    plain FP64 and FP32-storage plans of the same schedule timed back to back,
+    each with its normwise and worst-row relative error against the oracle."""
+    import spx
+    from spx import workloads as W
+
+    A = W.rail() if a.workload == "rail" else W.asic()
+    m, n, nnz = A["m"], A["n"], A["nnz"]
+    log(f"workload {a.workload}: m={m} n={n} nnz={nnz} mean_row={A['mean_row']:.2f} max_row={A['max_row']}")
+    x = W.vector(n, 31)
+    xd, yd = spx.DeviceBuffer.from_array(x), spx.DeviceBuffer(m * 8)
+    import oracle  # the checker
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    _, mag = oracle.spmv_dot2(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    res = {}
+    for prec, tag, sv in ((spx.PREC_F64, "fp64", 8), (spx.PREC_F32, "fp32", 4)):
+        P = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=a.kernel, items=a.items, parts=a.parts,
+                     sigma=a.sigma, precision=prec)
+        tot, ker, wall = P.time(xd, yd, 1.0, 0.0, warmup=a.warmup, reps=a.steps, wall=True)
+        P.execute(xd, yd)
+        spx.synchronize()
+        y = yd.download(np.float64, m)
+        B = W.bytes_model(m, n, nnz, sv)
+        row = np.abs(y - ref) / np.maximum(mag, 1e-300)
+        res[tag] = {"ms_per_step": round(wall, 5), "kernel_ms": round(ker, 5), "GBs": round(B / (wall * 1e-3) / 1e9, 2),
+                    "kernel_GBs": round(B / (ker * 1e-3) / 1e9, 2),
+                    "normwise_rel_err": float(np.linalg.norm(y - ref) / np.linalg.norm(ref)),
+                    "max_row_err_over_abs_sum": float(row.max())}
+        log(f"{tag}: wall {wall:.4f} ms, kernel {ker:.4f} ms, err {res[tag]['normwise_rel_err']:.3e}")
+        P.close()
+    f32 = res["fp32"]
+    print(json.dumps({
+        "metric": f"precision form on the {a.workload}-like matrix: FP32 storage vs plain FP64",
+        "value": f32["GBs"], "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": f32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 storage, f64 accumulation", "data": "synthetic",
+        "config": {"workload": f"{a.workload}-like CSR {m}x{n}, nnz={nnz}", "kernel": a.kernel},
+        "fp64": res["fp64"], "fp32": f32, "speedup_fp32_vs_fp64": round(res["fp64"]["ms_per_step"] / f32["ms_per_step"], 3),
+    }), flush=True)
+
+
+def run_spmm(a):
+    """Multi-RHS SpMM on the circuit-like matrix: the fused passes against
+    looping the single-vector execute, k columns, column-major X and Y."""
+    import ctypes as C
+
+    import spx
+    from spx import workloads as W
+
+    A = W.circuit(window=a.window)
+    m, n, nnz, k = A["m"], A["n"], A["nnz"], a.k
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=4, items=a.items, flags=spx.FLAG_FUSED_SPMM)
+    X = W.vector(n * k, 5)
+    xd, yd = spx.DeviceBuffer.from_array(X), spx.DeviceBuffer(m * k * 8)
+    L = spx.lib()
+
+    def timeit(fn):
+        for _ in range(a.warmup):
+            fn()
+        spx.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        spx.synchronize()
+        return (time.perf_counter() - t) * 1e3 / a.steps
+
+    fused = timeit(lambda: P.execute_multi(k, xd, n, yd, m))
+    yf = yd.download(np.float64, m * k)
+
+    def loop():
+        for j in range(k):
+            spx.check(L.spx_plan_execute(P.h, 1.0, C.c_void_p(xd.ptr + j * n * 8), 0.0,
+                                         C.c_void_p(yd.ptr + j * m * 8), None))
+
+    looped = timeit(loop)
+    yl = yd.download(np.float64, m * k)
+    B = 4 * (m + 1) + 12 * nnz + 8 * n * k + 8 * m * k
+    print(json.dumps({
+        "metric": f"fused SpMM, k={k} right-hand sides, vs looping the SpMV", "value": round(B / (fused * 1e-3) / 1e9, 2),
+        "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(fused, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"circuit-like CSR {m}x{n}, nnz={nnz}, X {n}x{k} and Y {m}x{k} column-major"},
+        "looped_ms": round(looped, 4), "speedup_vs_loop": round(looped / fused, 3),
+        "rel_fused_vs_loop": float(np.linalg.norm(yf - yl) / np.linalg.norm(yl)),
+    }), flush=True)
+
+
 def run_batch(a):
     """The batched form on 512 uneven matrices (n ~ lognormal(ln 14142, 1.32)
     clipped to [1e3, 2e5], Zipf(1.3) rows on [1, min(2048, n)], values
@@ -194,8 +283,10 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
-    ap.add_argument("--workload", choices=["circuit", "batch"], default="circuit",
-                    help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain")
+    ap.add_argument("--workload", choices=["circuit", "batch", "rail", "asic", "spmm"], default="circuit",
+                    help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain; "
+                         "rail / asic: the precision form (FP32 storage) vs plain FP64; spmm: fused multi-RHS")
+    ap.add_argument("--k", type=int, default=8, help="spmm: right-hand sides")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--autotune-top", type=int, default=4,
                     help="batch: schedules of the selector's shortlist the autotuner times")
@@ -207,6 +298,12 @@ def main():
 
     if a.workload == "batch":
         run_batch(a)
+        return
+    if a.workload in ("rail", "asic"):
+        run_precision(a)
+        return
+    if a.workload == "spmm":
+        run_spmm(a)
         return
 
     import spx
