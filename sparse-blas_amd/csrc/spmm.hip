@@ -6,21 +6,22 @@
 //  - k_xpose copies the pass's columns of X into XR, row-major with NC
 //    entries per column index (the extra entries zero), so the NC values one
 //    nonzero needs are one contiguous run (64 B at NC = 8);
-//  - k_seg_multi walks the units of the single-column engine (seg.hip).  The
-//    lanes of a wave form NC column groups of G = 64/NC lanes (lane = j*G + g):
-//    lane (j, g) handles column j of element group g, 16 consecutive nonzeros
-//    (two 8-element slots of the layout), so a chunk is 4*G groups =
-//    4096/NC nonzeros.  The NC lanes of a group load the same columns and
-//    values (one broadcast), and each of its 16 gathers from XR touches G
-//    contiguous 8*NC-byte runs per wave;
-//  - the group forms running sums of its products that restart at its start
-//    bits, exactly as the single-column engine does; a segmented scan over the G groups of
-//    each column (DPP within the column's lanes, wave totals through LDS)
-//    joins the pieces that cross groups, and the open sum of each column
-//    carries to the next chunk in its lanes;
-//  - the sums a chunk closes (at most 4096/NC + 1 per column) are gathered in
-//    LDS and stored in one coalesced pass: consecutive rows of one column of
-//    Y, or consecutive (slice, column) partials.
+//  - k_seg_multi walks the units of the single-column engine (seg.hip).  Each
+//    lane takes CPL = min(NC, 4) columns of an element group of 8 consecutive
+//    nonzeros; the lanes of a wave form NC/CPL column blocks of G groups
+//    (lane = block*G + group), so a chunk is 4*G groups of 8 nonzeros.  One
+//    load of the group's columns and values and one 8*CPL-byte read of an XR
+//    row serve CPL products;
+//  - per column the lane forms running sums of its products that restart at
+//    its start bits, exactly as the single-column engine does; a segmented
+//    scan over the block's G groups (DPP within the block's lanes, for the
+//    CPL columns at once; wave totals through LDS) joins the pieces that
+//    cross groups, and the open sum of each column carries to the next chunk
+//    in its lane;
+//  - the sums a chunk closes are gathered in LDS (up to CH/2 + 8 outputs; a
+//    chunk that closes more stores each where it closes) and stored in one
+//    coalesced pass: consecutive rows of one column of Y, or consecutive
+//    (slice, column) partials.
 // Long rows: one partial per (slice, column) at partial[slice * NC + j],
 // folded per column by k_long_reduce_multi.  Every addition happens in a
 // fixed order, so results are bitwise reproducible.

@@ -119,7 +119,7 @@ struct SegArgs {
 };
 
 // One multi-RHS pass (spmm.hip): columns of X and Y starting at the given
-// pointers, kk <= 32 of them.
+// pointers, kk <= kSpmmPass of them.
 constexpr int kSpmmPass = 16;  // right-hand sides per fused SpMM pass (A read once per pass)
 
 struct SpmmArgs {
