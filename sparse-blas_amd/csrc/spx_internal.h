@@ -13,7 +13,7 @@ constexpr int kBlock = 256;   // threads per workgroup (4 wave64s)
 constexpr int kWave = 64;     // CDNA wavefront width
 constexpr int kOmega = 64;    // lanes per CSR5-style tile
 constexpr int kXcd = 8;       // accelerator complex dies (one L2 each)
-constexpr int kColBlock = 2048;   // long-row path: column block width (x staged in LDS)
+constexpr int kColBlock = 2560;   // long-row path: column block width (x staged in LDS)
 constexpr int kMaxLong = 1024;    // long-row path: most long rows per plan
 
 // Workgroups for a direct launch over `count` units: a multiple of kXcd so
