@@ -268,7 +268,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel", type=int, default=4)
     ap.add_argument("--parts", type=int, default=1)
-    ap.add_argument("--items", type=int, default=4)
+    ap.add_argument("--items", type=int, default=2)
     ap.add_argument("--sigma", type=int, default=0)
     ap.add_argument("--window", type=int, default=1024,
                     help="half-width of the regular rows' column band (0 = uniform columns)")

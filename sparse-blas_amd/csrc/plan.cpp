@@ -619,7 +619,7 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     if (P->long_nnz + 8 * (int64_t)P->nblocks * P->nlong + 64 >= INT32_MAX)
       return fail(SPX_ERANGE, "padded long-row nonzeros must stay below 2^31 per plan");
     std::vector<int32_t> lsp, lseg;
-    seg_long(H, lrp, lcol, lval, P->nlong, P->nblocks, std::max<int64_t>(seg_target, 4 * kSegBlock * 8), lsp, lseg);
+    seg_long(H, lrp, lcol, lval, P->nlong, P->nblocks, std::max<int64_t>(seg_target, kLongUnit), lsp, lseg);
     P->nslong = (int64_t)H.units.size();
     P->nlseg = (int64_t)lseg.size();
     int rc;

@@ -14,6 +14,10 @@ constexpr int kWave = 64;     // CDNA wavefront width
 constexpr int kOmega = 64;    // lanes per CSR5-style tile
 constexpr int kXcd = 8;       // accelerator complex dies (one L2 each)
 constexpr int kColBlock = 2560;   // long-row path: column block width (x staged in LDS)
+// Long-row slices are cut into units per column block of at least this many
+// nonzeros, so a block is one unit (its x window staged once) unless it is
+// denser than that.
+constexpr int kLongUnit = 32768;
 constexpr int kMaxLong = 1024;    // long-row path: most long rows per plan
 
 // Workgroups for a direct launch over `count` units: a multiple of kXcd so
