@@ -115,7 +115,7 @@ struct spx_plan {
   double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials
   int64_t nxr = 0, npartk = 0;
   bool fused_spmm = false;
-  uint16_t *d_scol16 = nullptr;
+  uint32_t *d_scol12 = nullptr;  // staged units' 12-bit column offsets, 3 words per 8 elements
   int32_t *d_scol32 = nullptr;
   double *d_sval = nullptr;
   float *d_sval32 = nullptr;
@@ -143,7 +143,7 @@ struct spx_plan {
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
                     (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_xr, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
-                    (void *)d_scol16, (void *)d_scol32, (void *)d_sval, (void *)d_sval32})
+                    (void *)d_scol12, (void *)d_scol32, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
     for (auto e : done) (void)hipEventDestroy(e);
@@ -433,10 +433,10 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   ranges(0, P->nsstaged, xr.data() + (kXcd + 1));
   ranges(P->nsstaged, P->nsunits, xr.data() + 2 * (kXcd + 1));
   // the device order of the columns and values: values lane-fastest per wave
-  // block; columns of staged units as 16-bit window offsets in element
-  // order (a lane's 8 are one 16-byte load), those of other units in full,
-  // lane-fastest, packed unit after unit into col32 from U.cmin on
-  std::vector<uint16_t> c16((size_t)total, 0);
+  // block; columns of staged units as 12-bit window offsets in element
+  // order (a lane's 8 are three consecutive words), those of other units in
+  // full, lane-fastest, packed unit after unit into col32 from U.cmin on
+  std::vector<uint32_t> c12((size_t)(total >> 3) * 3, 0);  // total is a multiple of 8
   std::vector<int32_t> c32;
   {
     std::vector<double> pv((size_t)total, 0.0);
@@ -444,7 +444,14 @@ static int seg_upload(spx_plan *P, SegHost &H) {
       const int64_t e8 = ((int64_t)U.e1 + 7) & ~(int64_t)7;
       for (int64_t e = U.e0; e < e8; e++) pv[(size_t)seg_pos8(e, U.e0, U.e1)] = H.val[(size_t)e];
       if (U.flags & SEG_STAGE) {
-        for (int64_t e = U.e0; e < U.e1; e++) c16[(size_t)e] = (uint16_t)(H.col[(size_t)e] - U.cmin);
+        for (int64_t e = U.e0; e < U.e1; e++) {
+          const uint32_t v = (uint32_t)(H.col[(size_t)e] - U.cmin);  // < kSegStage <= 4096
+          const int64_t bit = (e & 7) * 12;
+          const size_t w = (size_t)(e >> 3) * 3 + (size_t)(bit >> 5);
+          const int sh = (int)(bit & 31);
+          c12[w] |= v << sh;
+          if (sh > 20) c12[w + 1] |= v >> (32 - sh);
+        }
       } else {
         const size_t base = c32.size();
         if (base + (size_t)(e8 - U.e0) >= ((size_t)1 << 31)) return fail(SPX_ERANGE, "too many unstaged nonzeros");
@@ -470,7 +477,7 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   if ((rc = P->upload(&P->d_sunits, H.units.data(), P->nsunits))) return rc;
   if ((rc = P->upload(&P->d_sxr, xr.data(), 3 * (kXcd + 1)))) return rc;
   if ((rc = P->upload(&P->d_sstart, H.start.data(), (int64_t)H.start.size()))) return rc;
-  if ((rc = P->upload(&P->d_scol16, c16.data(), total))) return rc;
+  if ((rc = P->upload(&P->d_scol12, c12.data(), (int64_t)c12.size()))) return rc;
   if (!c32.empty() && (rc = P->upload(&P->d_scol32, c32.data(), (int64_t)c32.size()))) return rc;
   if (P->precision == SPX_PREC_F32) {
     std::vector<float> v32(H.val.begin(), H.val.end());
@@ -1002,11 +1009,11 @@ static hipError_t launch_seg(spx_plan *P, int cls, const double *x, double alpha
   const int32_t *xr = P->d_sxr + cls * (kXcd + 1);
   const bool stage = cls == SEG_STAGED || (cls == SEG_ALL && P->sstage);
   if (P->precision == SPX_PREC_F32) {
-    SegArgs<float, XPair> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval32,
+    SegArgs<float, XPair> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval32,
                                P->d_xhl, alpha, beta, y, P->d_partial};
     return launch_seg_f32(a, stage, st);
   }
-  SegArgs<double, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol16, P->d_scol32, P->d_sval,
+  SegArgs<double, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval,
                                x, alpha, beta, y, P->d_partial};
   return launch_seg_f64(a, stage, st);
 }
@@ -1107,7 +1114,7 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
       a.nunits = p->nsunits;
       a.xr = p->d_sxr;
       a.start = p->d_sstart;
-      a.col16 = p->d_scol16;
+      a.col12 = p->d_scol12;
       a.col32 = p->d_scol32;
       a.val = p->d_sval;
       a.X = X + (size_t)j0 * ldx;

@@ -102,16 +102,20 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
         int32_t ci[LE];
         V vv[LE];
         if (nv > 0) {  // groups of 8 are padded: a lane's 8 slots are always readable
+          // The values and start bits are loaded first and the columns last:
+          // unpacking the 12-bit offsets waits for their load, and that wait
+          // then finds the other loads of the chunk already in flight.
           const int B = c + wid * kSegWaveBlock;  // this wave's block
           const int L = seg_lanes(B, U.e1);
+          Vec8<V>::load(a.val, B, L, lane, vv);
+          const uint32_t sb = __builtin_nontemporal_load(&a.start[ea >> 3]);
+          const uint32_t sn = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]);  // always padded in
           if (stg)
-            load8u16(a.col16 + ea, ci);  // window offsets
+            load8u12(a.col12, ea, ci);  // window offsets
           else
             load8i(a.col32 + U.cmin, B - U.e0, L, lane, ci);
-          Vec8<V>::load(a.val, B, L, lane, vv);
-          fb = __builtin_nontemporal_load(&a.start[ea >> 3]);
-          if (nv < LE) fb &= (1u << nv) - 1;
-          if (ea + nv < U.e1) tclosed = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]) & 1;
+          fb = nv < LE ? sb & ((1u << nv) - 1) : sb;
+          tclosed = ea + nv < U.e1 ? (int)(sn & 1) : 1;
         } else {
 #pragma unroll
           for (int i = 0; i < LE; i++) {

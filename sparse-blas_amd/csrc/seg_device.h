@@ -41,14 +41,20 @@ __device__ inline void load8i(const int32_t *p, int64_t B, int L, int lane, int3
   o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
 }
 
-// 8 consecutive 16-bit column offsets (16-byte aligned), widened to int.
-__device__ inline void load8u16(const uint16_t *p, int32_t *o) {
-  const vi4 u = __builtin_nontemporal_load(reinterpret_cast<const vi4 *>(p));
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    o[2 * k] = (int32_t)((uint32_t)u[k] & 0xffffu);
-    o[2 * k + 1] = (int32_t)((uint32_t)u[k] >> 16);
-  }
+// The 12-bit column offsets of the 8 elements from ea (a multiple of 8):
+// three consecutive 32-bit words, unpacked to int.
+__device__ inline void load8u12(const uint32_t *col12, int ea, int32_t *o) {
+  const uint32_t *p = col12 + (int64_t)(ea >> 3) * 3;
+  const uint32_t d0 = __builtin_nontemporal_load(p), d1 = __builtin_nontemporal_load(p + 1),
+                 d2 = __builtin_nontemporal_load(p + 2);
+  o[0] = (int32_t)(d0 & 0xfffu);
+  o[1] = (int32_t)((d0 >> 12) & 0xfffu);
+  o[2] = (int32_t)(__builtin_amdgcn_alignbit(d1, d0, 24) & 0xfffu);
+  o[3] = (int32_t)((d1 >> 4) & 0xfffu);
+  o[4] = (int32_t)((d1 >> 16) & 0xfffu);
+  o[5] = (int32_t)(__builtin_amdgcn_alignbit(d2, d1, 28) & 0xfffu);
+  o[6] = (int32_t)((d2 >> 8) & 0xfffu);
+  o[7] = (int32_t)(d2 >> 20);
 }
 
 // One step of a wave64 inclusive scan with DPP (data-parallel primitives:

@@ -120,17 +120,20 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
           const int B = U.e0 + (off & ~(kSegWaveBlock - 1));  // its wave block and slot
           const int vl = (off & (kSegWaveBlock - 1)) >> 3;
           const int L = seg_lanes(B, U.e1);
-          if (U.flags & SEG_STAGE) {  // 16-bit offsets from cmin
-            load8u16(a.col16 + ea, ci);
+          // values and start bits first: unpacking the columns waits for
+          // their load, with the other loads of the group already in flight
+          Vec8<double>::load(a.val, B, L, vl, vv);
+          const uint32_t sb = __builtin_nontemporal_load(&a.start[ea >> 3]);
+          const uint32_t sn = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]);  // always padded in
+          if (U.flags & SEG_STAGE) {  // 12-bit offsets from cmin
+            load8u12(a.col12, ea, ci);
 #pragma unroll
             for (int i = 0; i < LE; i++) ci[i] += U.cmin;
           } else {
             load8i(a.col32 + U.cmin, B - U.e0, L, vl, ci);
           }
-          Vec8<double>::load(a.val, B, L, vl, vv);
-          fb = __builtin_nontemporal_load(&a.start[ea >> 3]);
-          if (nv < LE) fb &= (1u << nv) - 1;
-          if (ea + nv < U.e1) tclosed = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]) & 1;
+          fb = nv < LE ? sb & ((1u << nv) - 1) : sb;
+          tclosed = ea + nv < U.e1 ? (int)(sn & 1) : 1;
         } else {
 #pragma unroll
           for (int i = 0; i < LE; i++) {

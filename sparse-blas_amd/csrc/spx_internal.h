@@ -68,14 +68,16 @@ struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
 
 // ---- segmented engine (seg.hip) ----------------------------------------------
 constexpr int kSegStage = 3328;  // most x entries a unit stages in LDS (26 KiB)
+static_assert(kSegStage <= 4096, "staged column offsets are 12-bit");
 constexpr int kSegBlock = 256;   // threads per segmented-engine workgroup (4 wave64s)
 constexpr int kSegOut = 520;     // sums per chunk gathered in LDS before they are stored
 
 // One workgroup unit of the segmented engine: padded elements [e0, e1)
 // (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order.  A
 // SEG_STAGE unit's columns lie in [cmin, cmin + cwin) and are stored as
-// 16-bit offsets from cmin (col16); any other unit's columns are stored in
-// full in col32 from index cmin on.  Segment s0 + j writes output o0 + j: y
+// 12-bit offsets from cmin (col12: element e's offset in bits 12 (e & 7) ..
+// 12 (e & 7) + 11 of the three 32-bit words 3 (e >> 3) .. 3 (e >> 3) + 2);
+// any other unit's columns are stored in full in col32 from index cmin on.  Segment s0 + j writes output o0 + j: y
 // row o0 + j, or partial slot o0 + j in SEG_PARTIAL units.  Bit (e & 7) of
 // start[e >> 3] is set when element e begins a segment.
 struct SegUnit {
@@ -113,7 +115,7 @@ struct SegArgs {
   int64_t nunits;
   const int32_t *xr;     // XCD x owns units [xr[x], xr[x + 1]) (kXcd + 1 entries)
   const uint8_t *start;  // segment-start bits, one per padded element
-  const uint16_t *col16;  // staged units: column - cmin, natural element order
+  const uint32_t *col12;  // staged units: column - cmin, 12 bits each, 8 per 3 words
   const int32_t *col32;   // unstaged units: columns from cmin on, lane-fastest (seg_pos4 - e0)
   const V *val;
   const X *x;
@@ -131,7 +133,7 @@ struct SpmmArgs {
   int64_t nunits;
   const int32_t *xr;
   const uint8_t *start;  // segment-start bits, one byte per 8 elements
-  const uint16_t *col16;  // as in SegArgs
+  const uint32_t *col12;  // as in SegArgs
   const int32_t *col32;
   const double *val;
   const double *X;  // the pass's first column
