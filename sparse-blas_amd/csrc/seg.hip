@@ -51,10 +51,13 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
   for (int t = tid; t < kSegOut; t += kSegBlock) s_ok[t] = 0;
   __syncthreads();
   int par = 0;  // double-buffer slot of the wave totals, alternating per chunk
-  for (int iter = 0;; iter++) {
-    const int u = seg_unit(a.xr, iter);
-    if (u < 0) break;
-    const SegUnit U = a.units[u];
+  SegWalk w = seg_walk(a.xr);
+  // Each unit's descriptor is loaded one unit ahead, so no wait for it stands
+  // between one unit's last chunk and the loads of the next unit's x window.
+  int un = seg_unit_prefetch(a.units, w.u, w.end, lane);
+  for (; w.u < w.end; w.u += w.step) {
+    const SegUnit U = seg_unit_take(un);
+    un = seg_unit_prefetch(a.units, w.u + w.step, w.end, lane);
     const bool stg = STAGE && (U.flags & SEG_STAGE);
     const bool part = (U.flags & SEG_PARTIAL) != 0;
     if (stg) {

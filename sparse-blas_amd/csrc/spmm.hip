@@ -96,10 +96,11 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
   const double *xr = a.XR + j0;
   const uint64_t below = ~0ull >> (63 - g);  // lanes g' <= g of a column block, shifted to bit 0
   int par = 0, tag = 0;
-  for (int iter = 0;; iter++) {
-    const int u = seg_unit(a.xr, iter);
-    if (u < 0) break;
-    const SegUnit U = a.units[u];
+  SegWalk w = seg_walk(a.xr);
+  int un = seg_unit_prefetch(a.units, w.u, w.end, lane);  // the next unit's descriptor
+  for (; w.u < w.end; w.u += w.step) {
+    const SegUnit U = seg_unit_take(un);
+    un = seg_unit_prefetch(a.units, w.u + w.step, w.end, lane);
     const bool part = (U.flags & SEG_PARTIAL) != 0;
     double carry[CPL];  // per column: the sum still open at the end of the previous chunk
 #pragma unroll

@@ -61,16 +61,38 @@ __device__ inline int64_t next_unit(int *queue, int64_t iter, int64_t count, int
   return u;
 }
 
-// Unit for this workgroup's iteration `iter`, or -1 when done.  XCD x owns
-// units [xr[x], xr[x+1]), ranges the host balanced by work, and its
+// The segmented engine's units for this workgroup.  XCD x owns units
+// [xr[x], xr[x+1]), ranges the host balanced by work, and its
 // workgroups walk them round-robin in order, so neighbouring units (which
 // share x entries) run at the same time in that XCD's L2.  No atomics: a
 // shared counter serialises at memory once hundreds of workgroups pop it.
-__device__ inline int seg_unit(const int32_t *xr, int iter) {
+struct SegWalk {
+  int u, end, step;  // this workgroup's units: u, u + step, ... while below end
+};
+__device__ inline SegWalk seg_walk(const int32_t *xr) {
   const int x = blockIdx.x % kXcd;
-  const int per = gridDim.x / kXcd;
-  const int u = xr[x] + (int)(blockIdx.x / kXcd) + iter * per;
-  return u < xr[x + 1] ? u : -1;
+  return {xr[x] + (int)(blockIdx.x / kXcd), xr[x + 1], (int)(gridDim.x / kXcd)};
+}
+
+// A unit descriptor spread over a wave, one 32-bit field per lane (lane l
+// holds field l % 8).  Loaded a unit ahead into one vector register, it is
+// read out with readlane when the unit starts: the load has long completed
+// by then, so no wait for it delays the unit.
+static_assert(sizeof(SegUnit) == 8 * sizeof(int32_t), "SegUnit is 8 int32 fields");
+__device__ inline int seg_unit_prefetch(const SegUnit *units, int u, int end, int lane) {
+  return u < end ? reinterpret_cast<const int32_t *>(units + u)[lane & 7] : 0;
+}
+__device__ inline SegUnit seg_unit_take(int v) {
+  SegUnit U;
+  U.e0 = __builtin_amdgcn_readlane(v, 0);
+  U.e1 = __builtin_amdgcn_readlane(v, 1);
+  U.s0 = __builtin_amdgcn_readlane(v, 2);
+  U.ns = __builtin_amdgcn_readlane(v, 3);
+  U.o0 = __builtin_amdgcn_readlane(v, 4);
+  U.cmin = __builtin_amdgcn_readlane(v, 5);
+  U.cwin = __builtin_amdgcn_readlane(v, 6);
+  U.flags = __builtin_amdgcn_readlane(v, 7);
+  return U;
 }
 
 }  // namespace spx
