@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
   constexpr int LE = 8;
   constexpr int CH = kSegBlock * LE;
   constexpr int NW = kSegBlock / kWave;
-  __shared__ X s_x[STAGE ? kSegStage : 1];
+  __shared__ __attribute__((aligned(16))) X s_x[STAGE ? kSegStage + 2 : 2];
   __shared__ double s_wv[2][NW];  // per wave: scan value, closed flag, start-bit count
   __shared__ int s_ws[2][NW];
   __shared__ int s_wc[2][NW];
@@ -60,37 +60,25 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
     un = seg_unit_prefetch(a.units, w.u + w.step, w.end, lane);
     const bool stg = STAGE && (U.flags & SEG_STAGE);
     const bool part = (U.flags & SEG_PARTIAL) != 0;
+    int lead = 0;  // s_x[lead + k] holds window entry k (lead: 0 or 1)
     if (stg) {
-      // The window is loaded in two rounds of NX entries per thread, each
-      // round's loads issued together (two memory latencies per unit, and
-      // few enough registers to keep the kernel's occupancy).  The first
-      // round's loads are in flight while the previous unit's gathers from
-      // s_x finish.
-      constexpr int NX = STAGE ? (kSegStage + 2 * kSegBlock - 1) / (2 * kSegBlock) : 1;
-      X xv[NX];
-#pragma unroll
-      for (int i = 0; i < NX; i++) {
-        const int k = tid + i * kSegBlock;
-        if (k < U.cwin) xv[i] = a.x[U.cmin + k];
-      }
+      // The window goes straight from global memory to LDS, 16 bytes per lane
+      // (global_load_lds_dwordx4), all of it at once: no registers, one
+      // memory latency.  Pairs start at the first 16-byte aligned entry; a
+      // leading and a trailing single entry are copied through registers.
+      const X *src = a.x + U.cmin;
+      lead = (int)(((uintptr_t)src >> 3) & 1);
+      const int np = (U.cwin - lead) >> 1;
       __syncthreads();  // the previous unit's gathers from s_x are done
-#pragma unroll
-      for (int i = 0; i < NX; i++) {
-        const int k = tid + i * kSegBlock;
-        if (k < U.cwin) s_x[k] = xv[i];
+      for (int q0 = wid * kWave; q0 < np; q0 += kSegBlock) {
+        const int q = q0 + lane;
+        if (q < np)
+          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + lead + 2 * q),
+                                           (__attribute__((address_space(3))) void *)(s_x + 2 * lead + 2 * q0), 16, 0, 0);
       }
-      if (U.cwin > NX * kSegBlock) {
-#pragma unroll
-        for (int i = 0; i < NX; i++) {
-          const int k = tid + (NX + i) * kSegBlock;
-          if (k < U.cwin) xv[i] = a.x[U.cmin + k];
-        }
-#pragma unroll
-        for (int i = 0; i < NX; i++) {
-          const int k = tid + (NX + i) * kSegBlock;
-          if (k < U.cwin) s_x[k] = xv[i];
-        }
-      }
+      if (tid == 0 && lead && U.cwin > 0) s_x[lead] = src[0];
+      if (tid == kWave && ((U.cwin - lead) & 1)) s_x[lead + U.cwin - 1] = src[U.cwin - 1];
+      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the copies have landed in LDS
       __syncthreads();
     }
     double carry = 0.0;  // sum still open at the end of the previous chunk
@@ -107,9 +95,12 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
         if (nv > 0) {  // groups of 8 are padded: a lane's 8 slots are always readable
           // The values and start bits are loaded first and the columns last:
           // unpacking the 12-bit offsets waits for their load, and that wait
-          // then finds the other loads of the chunk already in flight.
+          // then finds the other loads of the chunk already in flight.  The
+          // wave issues them at raised priority (s_setprio), so waves with
+          // loads to send go ahead of waves in their scan and emit steps.
           const int B = c + wid * kSegWaveBlock;  // this wave's block
           const int L = seg_lanes(B, U.e1);
+          __builtin_amdgcn_s_setprio(2);
           Vec8<V>::load(a.val, B, L, lane, vv);
           const uint32_t sb = __builtin_nontemporal_load(&a.start[ea >> 3]);
           const uint32_t sn = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]);  // always padded in
@@ -117,6 +108,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
             load8u12(a.col12, ea, ci);  // window offsets
           else
             load8i(a.col32 + U.cmin, B - U.e0, L, lane, ci);
+          __builtin_amdgcn_s_setprio(0);
           fb = nv < LE ? sb & ((1u << nv) - 1) : sb;
           tclosed = ea + nv < U.e1 ? (int)(sn & 1) : 1;
         } else {
@@ -132,7 +124,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
 #pragma unroll
           for (int i = 0; i < LE; i++) {
             const int cc = i < nv ? ci[i] : 0;  // in [0, cwin) for real nonzeros
-            const double t = prod(vv[i], s_x, cc);
+            const double t = prod(vv[i], s_x + lead, cc);
             p[i] = i < nv ? t : 0.0;
           }
         } else {
