@@ -122,7 +122,9 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
           const int vl = (off & (kSegWaveBlock - 1)) >> 3;
           const int L = seg_lanes(B, U.e1);
           // values and start bits first: unpacking the columns waits for
-          // their load, with the other loads of the group already in flight
+          // their load, with the other loads of the group already in flight;
+          // issued at raised wave priority, as in k_seg
+          __builtin_amdgcn_s_setprio(2);
           Vec8<double>::load(a.val, B, L, vl, vv);
           const uint32_t sb = __builtin_nontemporal_load(&a.start[ea >> 3]);
           const uint32_t sn = __builtin_nontemporal_load(&a.start[(ea + LE) >> 3]);  // always padded in
@@ -133,6 +135,7 @@ __global__ __launch_bounds__(kSegBlock) void k_seg_multi(SpmmArgs a) {
           } else {
             load8i(a.col32 + U.cmin, B - U.e0, L, vl, ci);
           }
+          __builtin_amdgcn_s_setprio(0);
           fb = nv < LE ? sb & ((1u << nv) - 1) : sb;
           tclosed = ea + nv < U.e1 ? (int)(sn & 1) : 1;
         } else {

@@ -390,3 +390,37 @@ def test_long_row_path_precision_form():
         y, P = run_plan(A, x, kernel=kernel, long_row=100, precision=spx.PREC_F32)
         assert P.info()["nlong"] > 0
         assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-7
+
+
+class _At:
+    """A device address inside a DeviceBuffer (what Plan.execute reads)."""
+
+    def __init__(self, buf, offset_bytes):
+        self.ptr = buf.ptr + offset_bytes
+
+
+@pytest.mark.parametrize("precision", [spx.PREC_F64, spx.PREC_F32])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_kernel4_x_at_either_16_byte_phase(offset, precision):
+    """Kernel 4 copies each staged x window into LDS 16 bytes per lane from
+    the window's first 16-byte aligned entry, plus single leading and trailing
+    entries.  Which of those a window has depends on the address of x: here x
+    starts on a 16-byte boundary (offset 0) or 8 bytes past one (offset 1),
+    with windows starting at even and odd columns alike, and y must match the
+    oracle either way."""
+    A = W.circuit(m=200_000, total=2_000_000, n_dense=10, window=700, seed=8)
+    x = W.vector(A["n"], 31)
+    y0 = W.vector(A["m"], 32)
+    print(f"x phase case offset={offset} precision={precision} nnz={A['nnz']}", flush=True)
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=4, items=1, precision=precision)
+    assert P.info()["staged_units"] > 0
+    xb = spx.DeviceBuffer((A["n"] + 2) * 8)
+    xb.upload(np.concatenate([np.full(offset, np.nan), x, np.full(2 - offset, np.nan)]))
+    yd = spx.DeviceBuffer.from_array(y0)
+    P.execute(_At(xb, 8 * offset), yd, 0.5, -2.0)
+    y = yd.download(np.float64, A["m"])
+    if precision == spx.PREC_F64:
+        check_fp64(A, x, y, 0.5, -2.0, y0)
+    else:
+        ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, 0.5, -2.0, y0)
+        assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
