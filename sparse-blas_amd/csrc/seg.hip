@@ -63,9 +63,12 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
     int lead = 0;  // s_x[lead + k] holds window entry k (lead: 0 or 1)
     if (stg) {
       // The window goes straight from global memory to LDS, 16 bytes per lane
-      // (global_load_lds_dwordx4), all of it at once: no registers, one
-      // memory latency.  Pairs start at the first 16-byte aligned entry; a
-      // leading and a trailing single entry are copied through registers.
+      // (global_load_lds_dwordx4: lane l's 16 bytes land at the wave's LDS
+      // address + 16 l), all of it at once: no registers, one memory
+      // latency.  Pairs of 8-byte entries start at the first 16-byte
+      // aligned entry; a leading and a trailing single entry are copied
+      // through registers.
+      static_assert(sizeof(X) == 8, "x entries are 8 bytes (a double or an FP32 hi/lo pair)");
       const X *src = a.x + U.cmin;
       lead = (int)(((uintptr_t)src >> 3) & 1);
       const int np = (U.cwin - lead) >> 1;
@@ -78,7 +81,9 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
       }
       if (tid == 0 && lead && U.cwin > 0) s_x[lead] = src[0];
       if (tid == kWave && ((U.cwin - lead) & 1)) s_x[lead + U.cwin - 1] = src[U.cwin - 1];
-      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the copies have landed in LDS
+      // vmcnt(0), expcnt and lgkmcnt left open (gfx9 encoding): this wave's
+      // copies have landed in LDS; the barrier then covers every wave's
+      __builtin_amdgcn_s_waitcnt(0x0f70);
       __syncthreads();
     }
     double carry = 0.0;  // sum still open at the end of the previous chunk
