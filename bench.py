@@ -272,6 +272,8 @@ def main():
     ap.add_argument("--sigma", type=int, default=0)
     ap.add_argument("--window", type=int, default=1024,
                     help="half-width of the regular rows' column band (0 = uniform columns)")
+    ap.add_argument("--dense", type=int, default=200,
+                    help="dense rows of the circuit matrix (0: its regular rows alone)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=100)
     ap.add_argument("--uniform-reps", type=int, default=20,
@@ -310,7 +312,7 @@ def main():
     from spx import workloads as W
 
     t0 = time.time()
-    A = W.circuit(window=a.window)
+    A = W.circuit(window=a.window, n_dense=a.dense)
     x = W.vector(A["n"], 22)
     m, n, nnz = A["m"], A["n"], A["nnz"]
     log(f"workload: m={m} n={n} nnz={nnz} mean_row={A['mean_row']:.2f} max_row={A['max_row']} "
@@ -366,6 +368,7 @@ def main():
     # the drop-in path end to end: host arrays in, host y out (upload, plan,
     # SpMV, download, free per call), as the reference's callers use it
     dropin_ms = None
+    dropin_phases = None
     if a.dropin_reps > 0:
         yh = np.zeros(m)
         ts = []
@@ -376,7 +379,9 @@ def main():
             if rc != 0:
                 raise RuntimeError(f"spMV_mgpu_v1 returned {rc}")
         dropin_ms = float(np.median(ts[1:])) * 1e3
-        log(f"drop-in spMV_mgpu_v1 (kernel 2) end to end: {dropin_ms:.1f} ms per call")
+        dropin_phases = spx.dropin_last_times()
+        log(f"drop-in spMV_mgpu_v1 (kernel 2) end to end: {dropin_ms:.1f} ms per call; last call's phases: "
+            + ", ".join(f"{k} {v:.2f}" for k, v in dropin_phases.items()))
 
     if a.copy_mb > 0:
         nb = a.copy_mb << 20
@@ -415,7 +420,8 @@ def main():
         "uniform_columns": uniform,
         "dropin_end_to_end": None if dropin_ms is None else {
             "entry": "spMV_mgpu_v1(kernel=2), host arrays", "ms_per_call": round(dropin_ms, 2),
-            "GBs": round(B / (dropin_ms * 1e-3) / 1e9, 2)},
+            "GBs": round(B / (dropin_ms * 1e-3) / 1e9, 2),
+            "last_call_phases_ms": {k: round(v, 3) for k, v in dropin_phases.items()}},
         "ref_getB_gbps": round(W.ref_getB(m, nnz) / (wall_ms * 1e-3) / 1e9, 2),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -51,8 +51,10 @@ enum {
 /* Value storage. */
 enum {
   SPX_PREC_F64 = 0,      /* FP64 values, FP64 x */
-  SPX_PREC_F32 = 1       /* FP32 values, x as FP32 (hi, lo) pairs, FP64
-                            accumulation, carries and beta update */
+  SPX_PREC_F32 = 1       /* FP32 values; x stays FP64 in memory and each
+                            product splits it into FP32 (hi, lo) in
+                            registers; FP64 accumulation, carries and beta
+                            update */
 };
 
 typedef struct spx_plan_desc {
@@ -117,7 +119,10 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
  * [12] device bytes held [13] precision [14] queues [15] long rows
  * [16] long-row nonzeros [17] column blocks [18] carry runs
  * [19] long-row threshold [20] segmented units that stage their x window
- * [21] 1 when staged and unstaged units run as two concurrent kernels.
+ * [21] 1 when staged and unstaged units run as two concurrent kernels
+ * [22] host microseconds spent copying the input's columns and values to
+ * the device during the build (kernels 1-3) [23] host microseconds of the
+ * whole build.
  * Returns the count. */
 int spx_plan_info(spx_plan_t p, int64_t *out, int cap);
 
@@ -249,6 +254,13 @@ int spx_dropin_csr5_kernel(int m, int n, int nnz, double *alpha,
 int spx_get_row_from_index(int n, long long *a, long long idx);
 double spx_get_time(void);
 double spx_get_gpu_availble_mem(int ngpu);
+
+/* Phases of the calling thread's last spx_dropin_baseline / v1 / v2 call, in
+ * milliseconds of host time: [0] plan build (including the upload of the
+ * matrix) [1] of which the matrix upload [2] upload of x (and y when
+ * beta != 0) [3] SpMV execute [4] download of y [5] release of device
+ * memory [6] whole call.  Returns the count written (at most cap). */
+int spx_dropin_last_times(double *ms, int cap);
 
 /* Number of stream partitions the v1 drop-in uses at ngpu = 1 (default 1;
  * the schedule selector sets it).  Returns the previous value. */

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <limits>
@@ -20,6 +21,11 @@ namespace {
 
 std::atomic<long long> g_free_override{-1};
 std::atomic<int> g_v1_streams{1};
+thread_local double g_times[7] = {0, 0, 0, 0, 0, 0, 0};
+
+double ms_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
 
 // Footprint of one partition as the reference's guard counts it
 // (SB/src/dspmv_mgpu_v1.cu:107-112): values, columns, int32 row pointer, x, y.
@@ -38,15 +44,22 @@ int code_of(int rc) {
 
 // Host arrays in, host y out; one plan.
 int run_host(const spx_plan_desc &d, double alpha, const double *x, double beta, double *y) {
+  const auto t_all = std::chrono::steady_clock::now();
+  for (double &t : g_times) t = 0.0;
   spx_plan_t plan = nullptr;
+  auto t = std::chrono::steady_clock::now();
   int rc = spx_plan_create(&plan, &d);
+  g_times[0] = ms_since(t);
   if (rc != SPX_OK) {
     std::fprintf(stderr, "spx: plan creation failed: %s\n", spx_last_error());
     return code_of(rc);
   }
+  int64_t info[24] = {0};
+  if (spx_plan_info(plan, info, 24) >= 23) g_times[1] = (double)info[22] / 1000.0;
   double *dx = nullptr, *dy = nullptr;
   int ret = 0;
   if (d.m > 0) {
+    t = std::chrono::steady_clock::now();
     dx = (double *)spx_malloc((int64_t)std::max(d.n, 1) * 8);
     dy = (double *)spx_malloc((int64_t)d.m * 8);
     if (!dx || !dy) {
@@ -54,16 +67,25 @@ int run_host(const spx_plan_desc &d, double alpha, const double *x, double beta,
     } else if ((d.n > 0 && spx_memcpy_h2d(dx, x, (int64_t)d.n * 8) != SPX_OK) ||
                (beta != 0.0 && spx_memcpy_h2d(dy, y, (int64_t)d.m * 8) != SPX_OK)) {
       ret = 1;
-    } else if (spx_plan_execute(plan, alpha, dx, beta, dy, nullptr) != SPX_OK || spx_synchronize() != SPX_OK) {
+    }
+    g_times[2] = ms_since(t);
+    t = std::chrono::steady_clock::now();
+    if (ret == 0 &&
+        (spx_plan_execute(plan, alpha, dx, beta, dy, nullptr) != SPX_OK || spx_synchronize() != SPX_OK)) {
       std::fprintf(stderr, "spx: execute failed: %s\n", spx_last_error());
       ret = -1;
-    } else if (spx_memcpy_d2h(y, dy, (int64_t)d.m * 8) != SPX_OK) {
-      ret = 1;
     }
+    g_times[3] = ms_since(t);
+    t = std::chrono::steady_clock::now();
+    if (ret == 0 && spx_memcpy_d2h(y, dy, (int64_t)d.m * 8) != SPX_OK) ret = 1;
+    g_times[4] = ms_since(t);
   }
+  t = std::chrono::steady_clock::now();
   if (dx) spx_free(dx);
   if (dy) spx_free(dy);
   spx_plan_destroy(plan);
+  g_times[5] = ms_since(t);
+  g_times[6] = ms_since(t_all);
   return ret;
 }
 
@@ -81,6 +103,13 @@ bool common_args_ok(int m, int n, long long nnz, double *alpha, double *beta, do
 extern "C" {
 
 void spx_debug_set_free_mem(long long bytes) { g_free_override = bytes; }
+
+int spx_dropin_last_times(double *ms, int cap) {
+  if (!ms || cap <= 0) return 0;
+  const int c = std::min(cap, 7);
+  for (int i = 0; i < c; i++) ms[i] = g_times[i];
+  return c;
+}
 
 int spx_set_v1_streams(int parts) {
   int old = g_v1_streams.load();

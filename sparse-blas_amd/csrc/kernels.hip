@@ -4,14 +4,15 @@
 //
 // Work is cut into partitions (nnz-balanced or row blocks) and each partition
 // into workgroup "units".  A row whose nonzeros fall in more than one unit is
-// never written by a unit: every unit writes its piece of such a row into a
-// carry slot, and k_fixup sums each row's carries in slot order (nonzero
-// order) and applies alpha and the beta term once.  Rows a unit holds whole
-// are written directly.  Summation order is fixed, so results are bitwise
-// reproducible; no atomics touch y.
+// never written by a unit: every unit writes its piece of such a row into
+// one of its two carry slots, tagged as the row's first piece, its final
+// piece or a piece between, and k_slot_fix sums each row's pieces in slot
+// order (nonzero order) and applies alpha and the beta term once.  Rows a
+// unit holds whole are written directly.  Summation order is fixed, so
+// results are bitwise reproducible; no atomics touch y.
 //
 // beta == 0 never reads y (BLAS convention), so y may hold NaN on input.
-#include "spx_device.h"
+#include "seg_device.h"
 
 namespace spx {
 
@@ -31,6 +32,22 @@ __device__ inline int row_search(const int32_t *rowptr, int lo, int hi, int64_t 
 
 __device__ inline double y_update(double alpha, double beta, double sum, const double *y, int r) {
   return beta == 0.0 ? alpha * sum : alpha * sum + beta * y[r];
+}
+
+// A final piece sums its row: back over the row's middle pieces (and unused
+// slots) to its first piece, then forward in slot order.
+__device__ inline void slot_fix(const int32_t *__restrict__ crow, const double *__restrict__ cval, int64_t k,
+                                double alpha, double beta, double *__restrict__ y) {
+  const int32_t c = crow[k];
+  if (c == -1 || !((uint32_t)c & kSlotFinal)) return;
+  const int32_t r = c & kSlotRow;
+  int64_t j = k - 1;
+  while (j >= 0 && crow[j] != (r | kSlotFirst)) j--;
+  double s = 0.0;
+  for (int64_t i = j < 0 ? 0 : j; i < k; i++)
+    if (crow[i] != -1) s += cval[i];
+  s += cval[k];
+  y[r] = y_update(alpha, beta, s, y, r);
 }
 
 // Index of the last partition whose `off` member is <= g (parts sorted by off).
@@ -244,15 +261,6 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_split_x(const double *__restrict__ x, XPair *__restrict__ xhl, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    double v = x[i];
-    float h = (float)v;
-    xhl[i] = XPair{h, (float)(v - (double)h)};
-  }
-}
-
 __global__ void k_to_f32(const double *__restrict__ v, float *__restrict__ o, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) o[i] = (float)v[i];
@@ -277,21 +285,6 @@ __global__ void k_reset(int *q, int n) {
   if (i < n) q[i] = 0;
 }
 
-// Sum each carried row's slots in slot order.  Which slots hold which row is
-// fixed by the sparsity structure alone, so the plan records the runs once
-// (row, first slot, end slot) and one thread per run does the sum.  Slots
-// inside a run that belong to no row (-1) are skipped.
-__global__ void k_fixup(const Run *__restrict__ runs, int64_t nruns, const int32_t *__restrict__ crow,
-                        const double *__restrict__ cval, double alpha, double beta, double *__restrict__ y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nruns) return;
-  const Run R = runs[i];
-  double s = 0.0;
-  for (int64_t k = R.beg; k < R.end; k++)
-    if (crow[k] == R.row) s += cval[k];
-  y[R.row] = y_update(alpha, beta, s, y, R.row);
-}
-
 // ---- long rows: fold the per-column-block partial sums -----------------------
 // The segmented engine sweeps the long rows by column blocks and leaves one
 // partial sum per (column block, row) slice; lseg[lsp[r] .. lsp[r+1]) lists
@@ -299,18 +292,15 @@ __global__ void k_fixup(const Run *__restrict__ runs, int64_t nruns, const int32
 // fixed tree order, then y[row] = alpha * sum + beta * y[row] (`assign`), or
 // y[row] += alpha * sum when the regular pass has already written beta * y0
 // for the row (it is empty there).
-__global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong,
-                                                        const int32_t *__restrict__ lsp,
-                                                        const int32_t *__restrict__ lseg,
-                                                        const int32_t *__restrict__ rows, double alpha,
-                                                        double beta, int assign, double *__restrict__ y) {
+__device__ inline void long_reduce(const double *__restrict__ partial, const int32_t *__restrict__ lsp,
+                                   const int32_t *__restrict__ lseg, const int32_t *__restrict__ rows, int r,
+                                   double alpha, double beta, int assign, double *__restrict__ y) {
   // Up to U slices per thread per round: all their slot indices are loaded
   // first, then all the partials, so a row of up to U*kBlock slices costs two
   // dependent memory round trips.  A thread adds its slices in slot order.
   constexpr int U = 16;
   __shared__ double s_w[kBlock / kWave];
-  const int r = blockIdx.x, tid = threadIdx.x;
-  if (r >= nlong) return;
+  const int tid = threadIdx.x;
   const int j0 = lsp[r], j1 = lsp[r + 1];
   double s = 0.0;
   for (int j = j0 + tid; j < j1; j += U * kBlock) {
@@ -338,323 +328,692 @@ __global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict
   }
 }
 
-// ---- kernel 1: CSR-vector, L lanes per row --------------------------------------
-// A group of L lanes walks one row with stride L, then reduces with
-// cross-lane shuffles (Bell & Garland's CSR-vector, wave64 edition).  The two
-// rows split at the partition boundary go to the partition's carry slots.
+__global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong,
+                                                        const int32_t *__restrict__ lsp,
+                                                        const int32_t *__restrict__ lseg,
+                                                        const int32_t *__restrict__ rows, double alpha,
+                                                        double beta, int assign, double *__restrict__ y) {
+  if ((int)blockIdx.x < nlong) long_reduce(partial, lsp, lseg, rows, blockIdx.x, alpha, beta, assign, y);
+}
 
-template <typename V, typename X, int L>
-__global__ __launch_bounds__(kBlock) void k_vector(KArgs<V, X> a) {
+// The end of an execute of kernels 1-3 in one launch: workgroups below nlong
+// fold a long row each, the rest run the slot fix-up (disjoint rows).
+__global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ partial, int nlong,
+                                                   const int32_t *__restrict__ lsp, const int32_t *__restrict__ lseg,
+                                                   const int32_t *__restrict__ rows, int assign,
+                                                   const int32_t *__restrict__ crow, const double *__restrict__ cval,
+                                                   int64_t nslots, double alpha, double beta,
+                                                   double *__restrict__ y) {
+  if ((int)blockIdx.x < nlong) {
+    long_reduce(partial, lsp, lseg, rows, blockIdx.x, alpha, beta, assign, y);
+  } else {
+    const int64_t k = (int64_t)(blockIdx.x - nlong) * kBlock + threadIdx.x;
+    if (k < nslots) slot_fix(crow, cval, k, alpha, beta, y);
+  }
+}
+
+// ---- kernel 1: CSR-vector, L lanes per row --------------------------------------
+// Bell & Garland's CSR-vector for wave64: a group of L lanes walks a row.  Each
+// lane takes pairs of neighbouring nonzeros (one 16-byte value load, one
+// 8-byte column load) from the even index at or below the row start, so the
+// group covers 2L nonzeros per step, and a group holds R rows at once so
+// their loads are in flight together.  The group's sum is reduced with DPP
+// lane permutations.  A unit is G*R consecutive rows of one partition; the
+// partition's split rows go to its carry slots.  The matrix arrays are
+// padded by two entries, so a pair load past the last nonzero is readable.
+
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef int vi2 __attribute__((ext_vector_type(2)));
+
+template <typename V>
+struct Pair;
+template <>
+struct Pair<double> {
+  __device__ static inline void load(const double *p, int64_t k, double &a, double &b) {
+    const vd2 u = __builtin_nontemporal_load(reinterpret_cast<const vd2 *>(p + k));
+    a = u.x;
+    b = u.y;
+  }
+};
+template <>
+struct Pair<float> {
+  __device__ static inline void load(const float *p, int64_t k, float &a, float &b) {
+    const vf2 u = __builtin_nontemporal_load(reinterpret_cast<const vf2 *>(p + k));
+    a = u.x;
+    b = u.y;
+  }
+};
+
+// One DPP permutation of a double (all lanes have a source for these controls).
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over each aligned group of L lanes; every lane of the group gets it.
+template <int L>
+__device__ inline double group_sum(double s) {
+  if (L >= 2) s += dpp_d<0xb1>(s);   // quad_perm [1,0,3,2]
+  if (L >= 4) s += dpp_d<0x4e>(s);   // quad_perm [2,3,0,1]
+  if (L >= 8) s += dpp_d<0x141>(s);  // row_half_mirror: the other quad of 8
+  if (L >= 16) s += dpp_d<0x140>(s); // row_mirror: the other 8 of 16
+  if (L >= 32) s += __shfl_xor(s, 16, kWave);
+  if (L >= 64) s += __shfl_xor(s, 32, kWave);
+  return s;
+}
+
+template <typename V, int L, int R>
+__global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
   constexpr int G = kBlock / L;
+  __shared__ __attribute__((aligned(16))) double s_x[kSegStage + 2];
   __shared__ int s_slot;
-  const int g = threadIdx.x / L, l = threadIdx.x % L;
-  for (int64_t iter = 0;; iter++) {
-    int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
-    if (u >= a.unit_count) break;
-    const Unit U = a.units[a.unit_begin + u];
-    const Part q = a.parts[U.part];  // by value: y stores cannot alias it
-    const int64_t r = (int64_t)q.row_lo + (int64_t)U.local * G + g;
-    if (r < q.row_hi) {
-      int64_t b = a.rowptr[r];
-      if (b < q.s) b = q.s;
-      int64_t e = a.rowptr[r + 1];
-      if (e > q.e + 1) e = q.e + 1;
-      double sum = 0.0, sum2 = 0.0;
-      int64_t j = b + l;
-      for (; j + L < e; j += 2 * L) {
-        sum += prod(a.val[j], a.x, a.col[j]);
-        sum2 += prod(a.val[j + L], a.x, a.col[j + L]);
-      }
-      if (j < e) sum += prod(a.val[j], a.x, a.col[j]);
-      sum += sum2;
+  const int tid = threadIdx.x, g = tid / L, l = tid % L, lane = tid % kWave;
+  int64_t iter = 0;
+  int64_t u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+  int dv = desc_prefetch(a.vunits + a.unit_begin, u, a.unit_count, lane);
+  while (u < a.unit_count) {
+    const int dcur = dv;
+    const int64_t gu = a.unit_begin + u;
+    u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+    dv = desc_prefetch(a.vunits + a.unit_begin, u, a.unit_count, lane);
+    const int r_lo = desc_field(dcur, 0), r_hi = desc_field(dcur, 1), s0 = desc_field(dcur, 2),
+              e1 = desc_field(dcur, 3), cmin = desc_field(dcur, 4), cwin = desc_field(dcur, 5);
+    const bool stg = cwin > 0;
+    int lead = 0;
+    if (stg) {
+      __syncthreads();  // the previous unit's gathers from s_x are done
+      lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
+    }
+    // row bounds, clamped to the partition; fp bit 0: the row begins in the
+    // partition, bit 1: it ends in it
+    int b[R], e[R], fp[R];
 #pragma unroll
-      for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
-      if (l == 0) {
-        if (r == q.r0 && q.sflag)
-          put_carry(a.crow, a.cval, q.carry_off, (int)r, sum);
-        else if (r == q.r1 && q.eflag)
-          put_carry(a.crow, a.cval, q.cslot1, (int)r, sum);
-        else
-          a.y[r] = y_update(a.alpha, a.beta, sum, a.y, (int)r);
+    for (int k = 0; k < R; k++) {
+      const int r = r_lo + k * G + g;
+      b[k] = 0;
+      e[k] = 0;
+      fp[k] = 3;
+      if (r < r_hi) {
+        const int rb = a.rowptr[r], re = a.rowptr[r + 1];
+        b[k] = rb > s0 ? rb : s0;
+        e[k] = re < e1 ? re : e1;
+        fp[k] = (rb >= s0 ? 1 : 0) | (re <= e1 ? 2 : 0);
       }
     }
-    if (a.queue) __syncthreads();
+    if (stg) {
+      stage_wait();
+      __syncthreads();  // every wave's part of the window is in LDS
+    }
+    // first step for all R rows at once: pair (b >> 1) + l covers
+    // elements 2p, 2p+1; products outside [b, e) are dropped and their
+    // gathers read the window's (or x's) first entry.  The steps are
+    // compiled twice, gathering from LDS or from x, so neither becomes a
+    // generic (flat) load.
+    double sum[R];
+    auto steps = [&](auto gx) {
+      V v0[R], v1[R];
+      vi2 c[R];
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        const int p = (b[k] >> 1) + l;
+        const int q = 2 * p < e[k] ? p : (b[k] >> 1);
+        Pair<V>::load(a.val, 2 * (int64_t)q, v0[k], v1[k]);
+        c[k] = __builtin_nontemporal_load(reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)q));
+      }
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        const int j = 2 * ((b[k] >> 1) + l);
+        const bool ok0 = j >= b[k] && j < e[k], ok1 = j + 1 >= b[k] && j + 1 < e[k];
+        const double t0 = prod(v0[k], gx(c[k].x, ok0));
+        const double t1 = prod(v1[k], gx(c[k].y, ok1));
+        sum[k] = (ok0 ? t0 : 0.0) + (ok1 ? t1 : 0.0);
+      }
+    // rows longer than the first step
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      for (int p = (b[k] >> 1) + l + L; 2 * p < e[k]; p += L) {
+        V v0, v1;
+        Pair<V>::load(a.val, 2 * (int64_t)p, v0, v1);
+        const vi2 c = *reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)p);
+        const bool ok1 = 2 * p + 1 < e[k];
+        const double t0 = prod(v0, gx(c.x, true)), t1 = prod(v1, gx(c.y, ok1));
+        sum[k] += t0 + (ok1 ? t1 : 0.0);
+      }
+    }
+    };
+    if (stg)
+      steps([&](int c, bool ok) -> double { return s_x[lead + (ok ? c - cmin : 0)]; });
+    else
+      steps([&](int c, bool ok) -> double { return a.x[ok ? c : 0]; });
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const double t = group_sum<L>(sum[k]);
+      const int r = r_lo + k * G + g;
+      if (l == 0 && r < r_hi) {
+        if (fp[k] == 3) {
+          a.y[r] = y_update(a.alpha, a.beta, t, a.y, r);
+        } else {
+          // a row cut by the partition bounds: the unit's first or last row
+          const int64_t slot = 2 * gu + (r == r_lo ? 0 : 1);
+          a.crow[slot] = (int32_t)((uint32_t)r | ((fp[k] & 1) ? (uint32_t)kSlotFirst : 0u) |
+                                   ((fp[k] & 2) ? kSlotFinal : 0u));
+          a.cval[slot] = t;
+        }
+      }
+    }
   }
 }
 
 // ---- kernel 2: merge path ------------------------------------------------------
 // Merrill & Garland (SC'16): the merge of row ends and nonzero indices is cut
-// into equal pieces of kBlock*IPT items, one per workgroup; unit boundary
-// coordinates come from k_merge_coords at plan time.  A workgroup stages its
-// row ends and its products val*x[col] in LDS with coalesced loads, each
-// thread merges IPT consecutive items, and a wave-shuffle segmented scan
-// carries partial row sums across threads.
+// into equal pieces of kBlock*IPT items, one per unit (MUnit: the merge-path
+// coordinates at both ends, found at plan time).  A workgroup stages the
+// unit's row ends and its products val*x[col] in LDS with coalesced loads;
+// each thread merges IPT consecutive items from LDS; a block-wide segmented
+// scan (DPP steps across each wave, wave totals through LDS) joins the row
+// sums that cross threads.  The unit's row sums are gathered back in LDS and
+// leave in one coalesced pass.  A row cut by a unit boundary leaves pieces in
+// the units' carry slots (2u: the piece continuing a row begun earlier,
+// 2u+1: the piece open at the unit's end), tagged with kSlotFirst /
+// kSlotFinal, and k_merge_fix adds them.  Every slot is rewritten each
+// time, so no slot state carries over between calls.
 
-template <typename V, typename X, int IPT>
-__global__ __launch_bounds__(kBlock) void k_merge(KArgs<V, X> a) {
+template <int CTRL, int ROWS, int BANKS>
+__device__ inline void scan_step_vs(double &v, int &sg) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, BANKS, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, BANKS, false);
+  const int su = __builtin_amdgcn_update_dpp(0, sg, CTRL, ROWS, BANKS, false);
+  if (!sg) {
+    v += __hiloint2double(hi, lo);
+    sg |= su;
+  }
+}
+
+template <typename V, int IPT>
+__global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
   constexpr int TW = kBlock * IPT;
-  __shared__ double s_prod[TW];
-  __shared__ int32_t s_re[TW + 1];
-  __shared__ double s_wv[kBlock / kWave];
-  __shared__ int s_ws[kBlock / kWave];
+  constexpr int NW = kBlock / kWave;
+  // the x window, then (after every gather) the unit's products, then its row sums
+  constexpr int XS = (kSegStage + 2) > TW ? (kSegStage + 2) : TW;
+  __shared__ __attribute__((aligned(16))) double s_x[XS];
+  __shared__ int32_t s_re[TW];  // row ends rowptr[r0 + k + 1]
+  __shared__ double s_wv[NW];
+  __shared__ int s_ws[NW];
+  __shared__ int s_rb;  // rowptr[r0]
   __shared__ int s_slot;
+  double *const s_prod = s_x;
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-
-  for (int64_t iter = 0;; iter++) {
-    int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
-    if (u >= a.unit_count) break;
-    const Unit U = a.units[a.unit_begin + u];
-    const Part q = a.parts[U.part];  // by value: y stores cannot alias it
-    const Coord c0 = a.coords[q.coord_off + U.local];
-    const Coord c1 = a.coords[q.coord_off + U.local + 1];
-    const int i0 = c0.row, j0 = c0.nz;
-    const int nrows = c1.row - c0.row, nnzs = c1.nz - c0.nz;
-
-    // stage row ends and products (coalesced; gathers of x issued together)
+  int64_t iter = 0;
+  int64_t u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+  int dv = desc_prefetch(a.munits + a.unit_begin, u, a.unit_count, lane);
+  while (u < a.unit_count) {
+    const int dcur = dv;
+    const int64_t gu = a.unit_begin + u;
+    u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+    dv = desc_prefetch(a.munits + a.unit_begin, u, a.unit_count, lane);
+    const int i0 = desc_field(dcur, 0), j0 = desc_field(dcur, 1), r1 = desc_field(dcur, 2), j1 = desc_field(dcur, 3);
+    const int cmin = desc_field(dcur, 4), cwin = desc_field(dcur, 5);
+    const int nrows = r1 - i0, nnzs = j1 - j0;
+    const bool stg = cwin > 0;
+    int lead = 0;
+    // the x window, the row ends and the unit's nonzeros are all requested
+    // before the first wait
+    if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
+    if (tid == 0) s_rb = a.rowptr[i0];
     for (int k = tid; k < nrows; k += kBlock) s_re[k] = a.rowptr[i0 + k + 1];
-    {
-      int cidx[IPT];
-      V vv[IPT];
+    int ci[IPT];
+    V vv[IPT];
+    __builtin_amdgcn_s_setprio(2);
 #pragma unroll
-      for (int r = 0; r < IPT; r++) {
-        int k = tid + r * kBlock;
-        cidx[r] = k < nnzs ? a.col[j0 + k] : 0;
-        vv[r] = k < nnzs ? a.val[j0 + k] : (V)0;
-      }
+    for (int r = 0; r < IPT; r++) {
+      const int k = tid + r * kBlock;
+      ci[r] = k < nnzs ? __builtin_nontemporal_load(a.col + j0 + k) : 0;
+      vv[r] = k < nnzs ? __builtin_nontemporal_load(a.val + j0 + k) : (V)0;
+    }
+    __builtin_amdgcn_s_setprio(0);
+    double pv[IPT];
+    if (stg) {
+      stage_wait();
+      __syncthreads();  // the window is in LDS
 #pragma unroll
-      for (int r = 0; r < IPT; r++) {
-        int k = tid + r * kBlock;
-        if (k < nnzs) s_prod[k] = prod(vv[r], a.x, cidx[r]);
-      }
+      for (int r = 0; r < IPT; r++) pv[r] = prod(vv[r], s_x[lead + (tid + r * kBlock < nnzs ? ci[r] - cmin : 0)]);
+      __syncthreads();  // every gather is done: the products overwrite the window
+    } else {
+#pragma unroll
+      for (int r = 0; r < IPT; r++) pv[r] = prod(vv[r], a.x[ci[r]]);
+    }
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+      const int k = tid + r * kBlock;
+      if (k < nnzs) s_prod[k] = pv[r];
     }
     __syncthreads();
 
-    // this thread's merge items [d, dend)
+    // this thread's merge items [d, dend): binary search of the diagonal
     const int total = nrows + nnzs;
-    int d = tid * IPT;
-    if (d > total) d = total;
-    int dend = d + IPT;
-    if (dend > total) dend = total;
+    const int d = tid * IPT < total ? tid * IPT : total;
+    const int dend = d + IPT < total ? d + IPT : total;
     int lo = d - nnzs > 0 ? d - nnzs : 0, hi = d < nrows ? d : nrows;
     while (lo < hi) {
-      int piv = (lo + hi) >> 1;
+      const int piv = (lo + hi) >> 1;
       if (s_re[piv] <= j0 + (d - piv - 1))
         lo = piv + 1;
       else
         hi = piv;
     }
-    int it = lo, jt = d - lo;
-    const int first_row = it;
-    double run = 0.0, first_val = 0.0;
-    int has_end = 0;
-    for (int k = d; k < dend; k++) {
-      if (it < nrows && (jt >= nnzs || s_re[it] <= j0 + jt)) {
-        if (!has_end) {
-          first_val = run;
-          has_end = 1;
-        } else {
-          a.y[i0 + it] = y_update(a.alpha, a.beta, run, a.y, i0 + it);
+    const int row0 = lo, jt0 = d - lo;
+    // the thread's next IPT row ends and products, read together; the walk
+    // then runs in registers (k-th step: at most k of each consumed)
+    int re[IPT];
+    double pr[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+      re[k] = row0 + k < nrows ? s_re[row0 + k] : INT32_MAX;
+      pr[k] = jt0 + k < nnzs ? s_prod[jt0 + k] : 0.0;
+    }
+    int ir = 0, jr = 0;
+    double run = 0.0;
+    double out[IPT];
+    uint32_t em = 0;  // bit k: item d + k ends a row
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+      out[k] = 0.0;
+      int rk = re[0];
+      double pk = pr[0];
+#pragma unroll
+      for (int i = 1; i < IPT; i++) {
+        if (i <= k) {
+          rk = ir == i ? re[i] : rk;
+          pk = jr == i ? pr[i] : pk;
         }
-        run = 0.0;
-        it++;
-      } else {
-        run += s_prod[jt];
-        jt++;
+      }
+      if (d + k < dend) {
+        // a row end precedes nonzero jt when rowptr[row + 1] <= jt
+        if (jt0 + jr >= nnzs || rk <= j0 + jt0 + jr) {
+          out[k] = run;
+          em |= 1u << k;
+          run = 0.0;
+          ir++;
+        } else {
+          run += pk;
+          jr++;
+        }
       }
     }
 
-    // segmented inclusive scan c(t) = run(t) + (has_end(t) ? 0 : c(t-1))
+    // segmented inclusive scan c(t) = run(t) + (t ends a row ? 0 : c(t-1))
     double v = run;
-    int sg = has_end;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      double vu = __shfl_up(v, off, kWave);
-      int su = __shfl_up(sg, off, kWave);
-      if (lane >= off && !sg) {
-        v += vu;
-        sg |= su;
-      }
-    }
+    int sg = em != 0;
+    scan_step_vs<0x111, 0xf, 0xf>(v, sg);  // row_shr:1
+    scan_step_vs<0x112, 0xf, 0xf>(v, sg);  // row_shr:2
+    scan_step_vs<0x114, 0xf, 0xe>(v, sg);  // row_shr:4
+    scan_step_vs<0x118, 0xf, 0xc>(v, sg);  // row_shr:8
+    scan_step_vs<0x142, 0xa, 0xf>(v, sg);  // row_bcast:15 into rows 1, 3
+    scan_step_vs<0x143, 0xc, 0xf>(v, sg);  // row_bcast:31 into rows 2, 3
     if (lane == kWave - 1) {
       s_wv[wid] = v;
       s_ws[wid] = sg;
     }
-    __syncthreads();
-    double prev = 0.0;  // inclusive value at the end of the previous wave
-    for (int w = 0; w < wid; w++) prev = s_wv[w] + (s_ws[w] ? 0.0 : prev);
+    __syncthreads();  // also: every thread is done reading s_prod
+    double prev = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (w < wid) prev = s_wv[w] + (s_ws[w] ? 0.0 : prev);
     const double c_incl = v + (sg ? 0.0 : prev);
     double carry_in = __shfl_up(c_incl, 1, kWave);
     if (lane == 0) carry_in = prev;
 
-    const int64_t slot0 = q.carry_off + 1 + 2 * (int64_t)U.local;
-    if (has_end) {
-      const double tot = first_val + carry_in;
-      const int r = i0 + first_row;
-      if (first_row == 0 && a.rowptr[i0] < j0)
-        put_carry(a.crow, a.cval, slot0, r, tot);  // row began in an earlier unit
-      else
-        a.y[r] = y_update(a.alpha, a.beta, tot, a.y, r);
-    }
-    if (tid == kBlock - 1) {
-      // tail: nonzeros after the last row end belong to the row still open
-      const int tail_from = nrows > 0 ? s_re[nrows - 1] : j0;
-      const bool has_tail = c1.nz > tail_from;
-      const bool head_split = nrows > 0 && a.rowptr[i0] < j0;
-      if (nrows == 0) {
-        if (has_tail)
-          put_carry(a.crow, a.cval, slot0, c1.row, c_incl);
-        else
-          put_carry(a.crow, a.cval, slot0, -1, 0.0);
-        put_carry(a.crow, a.cval, slot0 + 1, -1, 0.0);
-      } else {
-        if (!head_split) put_carry(a.crow, a.cval, slot0, -1, 0.0);
-        put_carry(a.crow, a.cval, slot0 + 1, has_tail ? c1.row : -1, has_tail ? c_incl : 0.0);
+    // the unit's row sums, by local row, into LDS
+    {
+      int rr = row0;
+      double cin = carry_in;
+#pragma unroll
+      for (int k = 0; k < IPT; k++) {
+        if ((em >> k) & 1u) {
+          s_prod[rr] = out[k] + cin;
+          cin = 0.0;
+          rr++;
+        }
       }
     }
     __syncthreads();
+    const int rb = s_rb;
+    const bool head_split = nrows > 0 && rb < j0;  // row r0 began in an earlier unit and ends here
+    for (int k = tid; k < nrows; k += kBlock) {
+      const double sv = s_prod[k];
+      if (k == 0 && head_split) {
+        a.crow[2 * gu] = (int32_t)((uint32_t)i0 | kSlotFinal);
+        a.cval[2 * gu] = sv;
+      } else {
+        a.y[i0 + k] = y_update(a.alpha, a.beta, sv, a.y, i0 + k);
+      }
+    }
+    if (tid == kBlock - 1) {
+      // nonzeros after the unit's last row end belong to row r1, still open
+      int32_t hc = -1, tc = -1;
+      double hv = 0.0, tv = 0.0;
+      if (nrows == 0) {
+        if (nnzs > 0) {
+          if (rb >= j0) {
+            tc = i0 | kSlotFirst;  // row r0 = r1 begins in this unit
+            tv = c_incl;
+          } else {
+            hc = i0;  // a middle piece of a row spanning this unit
+            hv = c_incl;
+          }
+        }
+      } else if (j1 > s_re[nrows - 1]) {
+        tc = r1 | kSlotFirst;
+        tv = c_incl;
+      }
+      if (!head_split) {
+        a.crow[2 * gu] = hc;
+        a.cval[2 * gu] = hv;
+      }
+      a.crow[2 * gu + 1] = tc;
+      a.cval[2 * gu + 1] = tv;
+    }
+    __syncthreads();  // LDS is rewritten by the next unit
   }
 }
 
-// ---- kernel 3: CSR5-style tiles -------------------------------------------------
-// One wave64 per tile of 64 lanes x sigma nonzeros stored lane-fastest, so
-// every load instruction reads 64 consecutive entries.  Each lane walks its
-// sigma elements; bit k of the lane's flag word marks a row start.  Row ids
-// come from the tile pointer plus a wave prefix count of the flags (or, in
-// tiles flagged as covering empty rows, a binary search of the row pointer).
-// The open row at a lane's end is completed by a backward segmented scan of
-// the following lanes' leading sums (Liu & Vinter, ICS'15, recast for wave64
-// with ballots and shuffles instead of stored y/scansum offsets).
-
-template <typename V, typename X>
-__device__ inline void tile_emit(const KArgs<V, X> &a, const Part &q, int64_t lrow, double v) {
-  const int r = (int)(q.row_lo + lrow);
-  if (r == q.r0 && q.sflag)
-    put_carry(a.crow, a.cval, q.carry_off, r, v);
-  else if (r == q.r1 && q.eflag)
-    put_carry(a.crow, a.cval, q.cslot1, r, v);
-  else
-    a.y[r] = y_update(a.alpha, a.beta, v, a.y, r);
+__global__ void k_slot_fix(const int32_t *__restrict__ crow, const double *__restrict__ cval, int64_t nslots,
+                           double alpha, double beta, double *__restrict__ y) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nslots) slot_fix(crow, cval, k, alpha, beta, y);
 }
 
-template <typename V, typename X, int MAXS>
-__global__ __launch_bounds__(kBlock) void k_tile(KArgs<V, X> a) {
+__global__ void k_merge_units(const Coord *__restrict__ coords, int64_t units, MUnit *__restrict__ out) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  const Coord c0 = coords[u], c1 = coords[u + 1];
+  out[u] = MUnit{c0.row, c0.nz, c1.row, c1.nz, c0.nz, c1.nz, 0, 0};  // window fields: the nonzero range
+}
+
+// ---- kernel 3: CSR5-style tiles -------------------------------------------------
+// Liu & Vinter's CSR5 (ICS'15) recast for wave64.  Tiles of 64 lanes x sigma
+// nonzeros are stored lane-fastest (element k of lane l at k*64 + l), so each
+// load instruction of a wave reads 64 consecutive entries; bit k of lane l's
+// flag word marks a row start.  A workgroup takes a unit of consecutive
+// tiles of one partition in chunks of four, one tile per wave.  Per chunk
+// each lane forms running sums of its sigma products that restart at its
+// start bits; a block-wide segmented scan (DPP steps across each wave, the
+// four wave totals through LDS) joins the sums that cross lanes and tiles,
+// and the sum still open at the chunk's end carries to the next chunk in
+// registers.  So y_offset and scansum_offset of CSR5's descriptor are not
+// stored: the row of a segment is the tile pointer plus a prefix count of
+// start bits (a binary search of the row pointer in tiles that cover empty
+// rows).  Row sums leave through LDS in one coalesced pass per chunk.  The
+// unit's x window is staged in LDS once per unit.  Pieces of rows cut by the
+// unit or partition bounds go to the unit's two carry slots with their
+// first/final tags (k_slot_fix adds them).
+
+// upper_bound over the partition's clamped local row pointer, minus 1: the
+// local row holding local position pos.
+__device__ inline int tile_row_search(const int32_t *rowptr, int row_lo, int s, int nz, int lo, int hi, int pos) {
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    int v = rowptr[row_lo + mid] - s;
+    v = v < 0 ? 0 : (v > nz ? nz : v);
+    if (v <= pos)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - 1;
+}
+
+template <int CTRL, int ROWS, int BANKS>
+__device__ inline void scan_step_c(int &cn) {
+  cn += __builtin_amdgcn_update_dpp(0, cn, CTRL, ROWS, BANKS, false);
+}
+
+template <typename V, int MAXS>
+__global__ __launch_bounds__(kBlock) void k_tile(KArgs<V> a) {
+  constexpr int NW = kBlock / kWave;
+  __shared__ __attribute__((aligned(16))) double s_x[kSegStage + 2];
+  __shared__ double s_wv[2][NW];  // per wave, double-buffered: scan value, closed flag, start count, dirty
+  __shared__ int s_ws[2][NW];
+  __shared__ int s_wc[2][NW];
+  __shared__ int s_wd[2][NW];
+  __shared__ double s_out[kTileOut];  // row sums closed in the current chunk, by row
+  __shared__ uint8_t s_ok[kTileOut];
+  __shared__ int s_unit[4];  // the unit's first row and start bit; the row and start bit after it
+  __shared__ int s_kb[2];    // the chunk's first row (row sums gathered from it on)
   __shared__ int s_slot;
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
   const int sigma = a.sigma;
-  for (int64_t iter = 0;; iter++) {
-    int64_t u = next_unit(a.queue, iter, a.unit_count, &s_slot);
-    if (u >= a.unit_count) break;
-    const Unit U = a.units[a.unit_begin + u];
-    const Part q = a.parts[U.part];  // by value: y stores cannot alias it
-    const int64_t t = (int64_t)U.local * (kBlock / kWave) + wid;
-    if (t < q.ntiles) {
-      const int64_t nz = q.e - q.s + 1;
-      const uint32_t tpw = a.tile_ptr[q.tp_off + t];
-      const bool dirty = tpw >> 31;
-      const int64_t R0 = tpw & 0x7FFFFFFFu;
-      const int64_t R1 = a.tile_ptr[q.tp_off + t + 1] & 0x7FFFFFFFu;
-      const int64_t tstart = t * kOmega * sigma;
-      int64_t tend = tstart + (int64_t)kOmega * sigma;
-      if (tend > nz) tend = nz;
-      const uint32_t fl = a.tile_flags[(q.tile_off + t) * kOmega + lane];
-      // does a row start exactly where this tile ends?  (bit 0 of the next
-      // tile's lane 0; a row open at the partition end does not continue)
-      const bool next_starts = tend >= nz || (a.tile_flags[(q.tile_off + t + 1) * kOmega] & 1u);
-      // exclusive prefix count of row starts over lanes
-      int excl = __popc(fl);
-#pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        int o = __shfl_up(excl, off, kWave);
-        if (lane >= off) excl += o;
-      }
-      excl -= __popc(fl);
-      const int f00 = __shfl(fl, 0, kWave) & 1;
-      int64_t cur = R0 + excl + (f00 ? 0 : 1) - 1;  // row open at the lane start
-      const int64_t base = (q.tile_off + t) * kOmega * sigma;
-      double sum = 0.0, head = 0.0;
-      int seen = 0;
-      const int64_t lpos = tstart + (int64_t)lane * sigma;
-      // all column/value loads first, then all x gathers, then the
-      // flag-driven segmented sum out of registers (the tile storage is
-      // padded to whole tiles, so loads need no bound; products past nz do)
-      double p[MAXS];
-      {
-        int c[MAXS];
-        V v[MAXS];
-#pragma unroll
-        for (int k = 0; k < MAXS; k++) {
-          c[k] = 0;
-          v[k] = (V)0;
-          if (k < sigma) {
-            const int64_t idx = base + (int64_t)k * kOmega + lane;
-            c[k] = a.col[idx];
-            v[k] = a.val[idx];
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < MAXS; k++) p[k] = (k < sigma && lpos + k < nz) ? prod(v[k], a.x, c[k]) : 0.0;
-      }
+  for (int t = tid; t < kTileOut; t += kBlock) s_ok[t] = 0;
+  int par = 0;
+  int64_t iter = 0;
+  int64_t u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+  int dv = desc_prefetch(a.tunits + a.unit_begin, u, a.unit_count, lane);
+  while (u < a.unit_count) {
+    const int dcur = dv;
+    u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+    dv = desc_prefetch(a.tunits + a.unit_begin, u, a.unit_count, lane);
+    const int gt0 = desc_field(dcur, 0), nt = desc_field(dcur, 1), lt0 = desc_field(dcur, 2),
+              tp0 = desc_field(dcur, 3), nz = desc_field(dcur, 4), row_lo = desc_field(dcur, 5),
+              s0 = desc_field(dcur, 6), gu = desc_field(dcur, 7), cmin = desc_field(dcur, 8),
+              cwin = desc_field(dcur, 9), tfl = desc_field(dcur, 10), last_lr = desc_field(dcur, 11);
+    const bool split_head = tfl & kTuSplitHead, split_tail = tfl & kTuSplitTail;
+    const bool stg = cwin > 0;
+    int lead = 0;
+    __syncthreads();  // the previous unit is done with s_x and s_unit
+    if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
+    if (tid == 0) {
+      // the row open at the unit's start and the one open at its end
+      const uint32_t tpa = a.tile_ptr[tp0], tpb = a.tile_ptr[tp0 + nt];
+      s_unit[0] = (int)(tpa & 0x7FFFFFFFu);
+      s_unit[1] = (int)(a.tile_flags[(int64_t)gt0 * kOmega] & 1u);
+      s_unit[2] = (int)(tpb & 0x7FFFFFFFu);
+      s_unit[3] = (int)(a.tile_flags[(int64_t)(gt0 + nt) * kOmega] & 1u);
+    }
+    double carry = 0.0;  // the sum open at the end of the previous chunk
+    for (int c = 0; 4 * c < nt; c++, par ^= 1) {
+      const int ti = 4 * c + wid;  // this wave's tile within the unit
+      const bool tvalid = ti < nt;
+      const int64_t gt = (int64_t)gt0 + ti;
+      const int lt = lt0 + ti;
+      uint32_t fl = 0, fn = 0, tpw = 0, tpn = 0;
+      int32_t ci[MAXS];
+      V vv[MAXS];
 #pragma unroll
       for (int k = 0; k < MAXS; k++) {
-        if (k < sigma) {
-          if ((fl >> k) & 1u) {
-            if (seen)
-              tile_emit(a, q, cur, sum);
-            else
-              head = sum;
-            seen = 1;
-            sum = 0.0;
-            cur = dirty ? local_row_search(a.rowptr, q, R0, R1 + 1, lpos + k) : cur + 1;
-          }
-          sum += p[k];
-        }
+        ci[k] = 0;
+        vv[k] = (V)0;
       }
-      double tail = 0.0;
-      if (seen)
-        tail = sum;
-      else
-        head = sum;
-      // T(l) = head(l) + (seen(l) ? 0 : T(l+1)): backward segmented scan
-      double tv = head;
-      int ts = seen;
+      if (tvalid) {
+        __builtin_amdgcn_s_setprio(2);
+        const int64_t base = gt * kOmega * sigma + lane;
 #pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        double vd = __shfl_down(tv, off, kWave);
-        int sd = __shfl_down(ts, off, kWave);
-        if (lane + off < kWave && !ts) {
-          tv += vd;
-          ts |= sd;
-        }
-      }
-      double tnext = __shfl_down(tv, 1, kWave);
-      if (lane == kWave - 1) tnext = 0.0;
-      const uint64_t mask = __ballot(seen);
-      const int64_t slot0 = q.carry_off + 1 + 2 * t;
-      const bool later = lane < kWave - 1 && (mask >> (lane + 1)) != 0;
-      if (seen) {
-        const double tot = tail + tnext;
-        if (!later) {
-          // the last row started in this tile: carried on if it runs past the tile
-          if (!next_starts) {
-            put_carry(a.crow, a.cval, slot0 + 1, (int)(q.row_lo + cur), tot);
-          } else {
-            tile_emit(a, q, cur, tot);
-            put_carry(a.crow, a.cval, slot0 + 1, -1, 0.0);
+        for (int k = 0; k < MAXS; k++) {
+          if (k < sigma) {
+            vv[k] = __builtin_nontemporal_load(a.val + base + (int64_t)k * kOmega);
+            ci[k] = __builtin_nontemporal_load(a.col + base + (int64_t)k * kOmega);
           }
-        } else {
-          tile_emit(a, q, cur, tot);
+        }
+        fl = __builtin_nontemporal_load(a.tile_flags + gt * kOmega + lane);
+        fn = __builtin_nontemporal_load(a.tile_flags + gt * kOmega + lane + 1);  // padded by one tile
+        tpw = a.tile_ptr[tp0 + ti];
+        tpn = a.tile_ptr[tp0 + ti + 1];  // the tile's last row (a dirty tile's search range)
+        __builtin_amdgcn_s_setprio(0);
+      }
+      if (c == 0) {
+        if (stg) stage_wait();
+        __syncthreads();  // the window and the unit's facts are in LDS
+      }
+      const int lrs = s_unit[0], fs = s_unit[1];
+      // products of this lane's valid elements
+      const int lpos = lt * kOmega * sigma + lane * sigma;  // local position of the lane's element 0
+      int nv = nz - lpos;
+      nv = !tvalid || nv < 0 ? 0 : (nv > sigma ? sigma : nv);
+      double p[MAXS];
+      if (stg) {
+#pragma unroll
+        for (int k = 0; k < MAXS; k++) {
+          const bool ok = k < nv;
+          const double t = prod(vv[k], s_x[lead + (ok ? ci[k] - cmin : 0)]);
+          p[k] = ok ? t : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < MAXS; k++) {
+          const bool ok = k < nv;
+          const double t = prod(vv[k], a.x[ok ? ci[k] : 0]);
+          p[k] = ok ? t : 0.0;
         }
       }
-      if (lane == 0) {
-        // leading piece of a row begun in an earlier tile
-        if (!f00)
-          put_carry(a.crow, a.cval, slot0, (int)(q.row_lo + R0), tv);
-        else
-          put_carry(a.crow, a.cval, slot0, -1, 0.0);
-        if (mask == 0) put_carry(a.crow, a.cval, slot0 + 1, -1, 0.0);
+      const uint32_t vmask = nv >= 32 ? 0xFFFFFFFFu : (1u << nv) - 1u;  // this lane's valid elements
+      const uint32_t fb = fl & vmask;                                    // their start bits
+      const int tclosed = lpos + nv >= nz ? 1 : (int)(fn & 1u);     // the lane's last element ends a row
+      // running sums that restart at every start bit
+      const uint32_t inner = fb & ~1u;
+      const int f1 = inner ? __builtin_ctz(inner) : nv;
+#pragma unroll
+      for (int k = 1; k < MAXS; k++)
+        if (!((fb >> k) & 1u)) p[k] = p[k - 1] + p[k];
+      double last = p[0];
+#pragma unroll
+      for (int k = 1; k < MAXS; k++)
+        if (k < nv) last = p[k];
+      const int has_close = nv > 0 && (inner != 0 || tclosed);
+      const double run = nv > 0 && !tclosed ? last : 0.0;
+      const int cnt = __builtin_popcount(fb);
+      // segmented scan of the running sums across the workgroup, and a wave
+      // scan of start-bit counts (rows of this tile)
+      double v = run;
+      int sg = has_close, cn = cnt;
+      scan_step<0x111, 0xf, 0xf>(v, sg, cn);  // row_shr:1
+      scan_step<0x112, 0xf, 0xf>(v, sg, cn);  // row_shr:2
+      scan_step<0x114, 0xf, 0xe>(v, sg, cn);  // row_shr:4
+      scan_step<0x118, 0xf, 0xc>(v, sg, cn);  // row_shr:8
+      scan_step<0x142, 0xa, 0xf>(v, sg, cn);  // row_bcast:15 into rows 1, 3
+      scan_step<0x143, 0xc, 0xf>(v, sg, cn);  // row_bcast:31 into rows 2, 3
+      const bool dirty = tvalid && (tpw >> 31);
+      const int R0 = (int)(tpw & 0x7FFFFFFFu);
+      const int f00 = (int)(__builtin_amdgcn_readlane(fl, 0) & 1u);
+      if (lane == kWave - 1) {
+        s_wv[par][wid] = v;
+        s_ws[par][wid] = sg;
+        s_wc[par][wid] = cn;
+        s_wd[par][wid] = dirty ? 1 : 0;
+      }
+      if (tid == 0) s_kb[par] = R0;
+      __syncthreads();
+      double prev = carry, tot = carry;
+      int totc = 0, anyd = 0;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const double wv = s_wv[par][w];
+        const int ws = s_ws[par][w];
+        if (w < wid) prev = wv + (ws ? 0.0 : prev);
+        tot = wv + (ws ? 0.0 : tot);
+        totc += s_wc[par][w];
+        anyd |= s_wd[par][w];
+      }
+      const double c_incl = v + (sg ? 0.0 : prev);
+      double carry_in = __shfl_up(c_incl, 1, kWave);
+      if (lane == 0) carry_in = prev;
+      const int excl = cn - cnt;  // start bits in earlier lanes of this tile
+      // rows closed in this chunk lie in [kb, kb + totc] unless a tile
+      // covers empty rows; they go through LDS when that span fits
+      const int kb = s_kb[par];
+      const bool via_lds = !anyd && totc + 1 <= kTileOut;
+      const uint32_t ends = nv > 0 ? ((fb >> 1) | ((uint32_t)tclosed << (nv - 1))) & vmask : 0u;
+#pragma unroll
+      for (int k = 0; k < MAXS; k++) {
+        if ((ends >> k) & 1u) {
+          const double val = k < f1 ? p[k] + carry_in : p[k];
+          const int lr = dirty ? tile_row_search(a.rowptr, row_lo, s0, nz, R0, (int)(tpn & 0x7FFFFFFFu) + 1, lpos + k)
+                               : R0 + excl + __builtin_popcount(fb & ((2u << k) - 1u)) - f00;
+          const int r = row_lo + lr;
+          // the pieces cut by the unit's start or the partition's bounds go
+          // to carry slots, the rows complete here to y
+          if (lr == lrs && !fs) {
+            const bool fin = !(split_tail && lr == last_lr);
+            a.crow[2 * (int64_t)gu] = (int32_t)((uint32_t)r | (fin ? kSlotFinal : 0u));
+            a.cval[2 * (int64_t)gu] = val;
+          } else if (split_head && lr == 0) {
+            const bool fin = !(split_tail && lr == last_lr);
+            a.crow[2 * (int64_t)gu] = (int32_t)((uint32_t)r | (fin ? kSlotFinal : 0u));
+            a.cval[2 * (int64_t)gu] = val;
+          } else if (split_tail && lr == last_lr) {
+            a.crow[2 * (int64_t)gu + 1] = r | kSlotFirst;
+            a.cval[2 * (int64_t)gu + 1] = val;
+          } else if (via_lds) {
+            s_out[lr - kb] = val;
+            s_ok[lr - kb] = 1;
+          } else {
+            a.y[r] = y_update(a.alpha, a.beta, val, a.y, r);
+          }
+        }
+      }
+      if (via_lds) {
+        // s_out is written only after the next chunk's first barrier, which
+        // every thread reaches after this pass: one buffer suffices
+        __syncthreads();
+        for (int t = tid; t <= totc; t += kBlock) {
+          if (s_ok[t]) {
+            const int r = row_lo + kb + t;
+            a.y[r] = y_update(a.alpha, a.beta, s_out[t], a.y, r);
+            s_ok[t] = 0;
+          }
+        }
+      }
+      carry = tot;
+    }
+    // the sum open at the unit's end: the row holding the next tile's first
+    // element continues, unless that element starts a row or the partition
+    // ends here
+    if (tid == 0) {
+      const int64_t uend = (int64_t)(lt0 + nt) * kOmega * sigma;
+      if (uend < nz && !s_unit[3]) {
+        const int lr = s_unit[2];
+        const bool began = !(lr == s_unit[0] && !s_unit[1]);  // else the whole unit lies in the row
+        const bool first = began && !(split_head && lr == 0);
+        const int64_t slot = 2 * (int64_t)gu + (began ? 1 : 0);
+        a.crow[slot] = (row_lo + lr) | (first ? kSlotFirst : 0);
+        a.cval[slot] = carry;
       }
     }
-    if (a.queue) __syncthreads();
+  }
+}
+
+// ---- x windows of units (plan time) -----------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_unit_window(const int32_t *__restrict__ col, int32_t *__restrict__ f,
+                                                        int stride, int off, int64_t nunits) {
+  __shared__ int s_lo[kBlock / kWave], s_hi[kBlock / kWave];
+  const int64_t u = blockIdx.x;
+  if (u >= nunits) return;
+  const int lo = f[u * stride + off], hi = f[u * stride + off + 1];
+  int mn = INT32_MAX, mx = -1;
+  for (int k = lo + (int)threadIdx.x; k < hi; k += kBlock) {
+    const int c = col[k];
+    mn = c < mn ? c : mn;
+    mx = c > mx ? c : mx;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const int a2 = __shfl_xor(mn, o, kWave), b2 = __shfl_xor(mx, o, kWave);
+    mn = a2 < mn ? a2 : mn;
+    mx = b2 > mx ? b2 : mx;
+  }
+  if (threadIdx.x % kWave == 0) {
+    s_lo[threadIdx.x / kWave] = mn;
+    s_hi[threadIdx.x / kWave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / kWave; w++) {
+      mn = s_lo[w] < mn ? s_lo[w] : mn;
+      mx = s_hi[w] > mx ? s_hi[w] : mx;
+    }
+    const int64_t win = mx >= mn ? (int64_t)mx - mn + 1 : 0;
+    f[u * stride + off] = win > 0 && win <= kSegStage ? mn : 0;
+    f[u * stride + off + 1] = win > 0 && win <= kSegStage ? (int)win : 0;
   }
 }
 
@@ -707,12 +1066,6 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_split_x(const double *x, XPair *xhl, int64_t n, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_split_x, dim3(nblk(n)), dim3(kBlock), 0, st, x, xhl, n);
-  return hipGetLastError();
-}
-
 hipError_t launch_to_f32(const double *v, float *o, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(kBlock), 0, st, v, o, n);
@@ -728,13 +1081,6 @@ hipError_t launch_empty_rows(const int32_t *rows, int64_t n, double beta, double
 hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag, hipStream_t st) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_find_empty, dim3(nblk(m)), dim3(kBlock), 0, st, rowptr, m, flag);
-  return hipGetLastError();
-}
-
-hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow, const double *cval, double alpha,
-                        double beta, double *y, hipStream_t st) {
-  if (nruns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fixup, dim3(nblk(nruns)), dim3(kBlock), 0, st, runs, nruns, crow, cval, alpha, beta, y);
   return hipGetLastError();
 }
 
@@ -774,62 +1120,97 @@ static int launch_grid(int want, int cap) {
 
 #define RESIDENT(kern) ([] { static const int c = resident_cap((const void *)(kern)); return c; }())
 
-template <typename V, typename X>
-static hipError_t vec_dispatch(const KArgs<V, X> &a, int lanes, int grid, hipStream_t st) {
+template <typename V>
+static hipError_t vec_dispatch(const KArgs<V> &a, int lanes, int grid, hipStream_t st) {
+#define SPX_VEC(L, R)                                                                                   \
+  hipLaunchKernelGGL((k_vector<V, L, R>), dim3(launch_grid(grid, RESIDENT((k_vector<V, L, R>)))), dim3(kBlock), \
+                     0, st, a)
   switch (lanes) {
-    case 1: hipLaunchKernelGGL((k_vector<V, X, 1>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 1>)))), dim3(kBlock), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_vector<V, X, 2>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 2>)))), dim3(kBlock), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_vector<V, X, 4>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 4>)))), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_vector<V, X, 8>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 8>)))), dim3(kBlock), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((k_vector<V, X, 16>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 16>)))), dim3(kBlock), 0, st, a); break;
-    case 32: hipLaunchKernelGGL((k_vector<V, X, 32>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 32>)))), dim3(kBlock), 0, st, a); break;
-    case 64: hipLaunchKernelGGL((k_vector<V, X, 64>), dim3(launch_grid(grid, RESIDENT((k_vector<V, X, 64>)))), dim3(kBlock), 0, st, a); break;
+    case 2: SPX_VEC(2, kVecRows(2)); break;
+    case 4: SPX_VEC(4, kVecRows(4)); break;
+    case 8: SPX_VEC(8, kVecRows(8)); break;
+    case 16: SPX_VEC(16, kVecRows(16)); break;
+    case 32: SPX_VEC(32, kVecRows(32)); break;
+    case 64: SPX_VEC(64, kVecRows(64)); break;
     default: return hipErrorInvalidValue;
   }
+#undef SPX_VEC
   return hipGetLastError();
 }
 
-template <typename V, typename X>
-static hipError_t merge_dispatch(const KArgs<V, X> &a, int items, int grid, hipStream_t st) {
+template <typename V>
+static hipError_t merge_dispatch(const KArgs<V> &a, int items, int grid, hipStream_t st) {
+#define SPX_MERGE(I) \
+  hipLaunchKernelGGL((k_merge<V, I>), dim3(launch_grid(grid, RESIDENT((k_merge<V, I>)))), dim3(kBlock), 0, st, a)
   switch (items) {
-    case 2: hipLaunchKernelGGL((k_merge<V, X, 2>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 2>)))), dim3(kBlock), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_merge<V, X, 4>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 4>)))), dim3(kBlock), 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_merge<V, X, 6>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 6>)))), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_merge<V, X, 8>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 8>)))), dim3(kBlock), 0, st, a); break;
-    case 12: hipLaunchKernelGGL((k_merge<V, X, 12>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 12>)))), dim3(kBlock), 0, st, a); break;
-    case 16: hipLaunchKernelGGL((k_merge<V, X, 16>), dim3(launch_grid(grid, RESIDENT((k_merge<V, X, 16>)))), dim3(kBlock), 0, st, a); break;
+    case 2: SPX_MERGE(2); break;
+    case 4: SPX_MERGE(4); break;
+    case 6: SPX_MERGE(6); break;
+    case 8: SPX_MERGE(8); break;
+    case 12: SPX_MERGE(12); break;
+    case 16: SPX_MERGE(16); break;
     default: return hipErrorInvalidValue;
   }
+#undef SPX_MERGE
   return hipGetLastError();
 }
 
-hipError_t launch_vector_f64(const KArgs<double, double> &a, int lanes, int grid, hipStream_t st) {
+hipError_t launch_vector_f64(const KArgs<double> &a, int lanes, int grid, hipStream_t st) {
   return vec_dispatch(a, lanes, grid, st);
 }
-hipError_t launch_vector_f32(const KArgs<float, XPair> &a, int lanes, int grid, hipStream_t st) {
+hipError_t launch_vector_f32(const KArgs<float> &a, int lanes, int grid, hipStream_t st) {
   return vec_dispatch(a, lanes, grid, st);
 }
-hipError_t launch_merge_f64(const KArgs<double, double> &a, int items, int grid, hipStream_t st) {
+hipError_t launch_merge_f64(const KArgs<double> &a, int items, int grid, hipStream_t st) {
   return merge_dispatch(a, items, grid, st);
 }
-hipError_t launch_merge_f32(const KArgs<float, XPair> &a, int items, int grid, hipStream_t st) {
+hipError_t launch_merge_f32(const KArgs<float> &a, int items, int grid, hipStream_t st) {
   return merge_dispatch(a, items, grid, st);
 }
-template <typename V, typename X>
-static hipError_t tile_dispatch(const KArgs<V, X> &a, int grid, hipStream_t st) {
+template <typename V>
+static hipError_t tile_dispatch(const KArgs<V> &a, int grid, hipStream_t st) {
   if (a.sigma <= 8)
-    hipLaunchKernelGGL((k_tile<V, X, 8>), dim3(launch_grid(grid, RESIDENT((k_tile<V, X, 8>)))), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_tile<V, 8>), dim3(launch_grid(grid, RESIDENT((k_tile<V, 8>)))), dim3(kBlock), 0, st, a);
   else if (a.sigma <= 16)
-    hipLaunchKernelGGL((k_tile<V, X, 16>), dim3(launch_grid(grid, RESIDENT((k_tile<V, X, 16>)))), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_tile<V, 16>), dim3(launch_grid(grid, RESIDENT((k_tile<V, 16>)))), dim3(kBlock), 0, st, a);
   else
-    hipLaunchKernelGGL((k_tile<V, X, 32>), dim3(launch_grid(grid, RESIDENT((k_tile<V, X, 32>)))), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_tile<V, 32>), dim3(launch_grid(grid, RESIDENT((k_tile<V, 32>)))), dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
-hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st) {
+hipError_t launch_tile_f64(const KArgs<double> &a, int grid, hipStream_t st) {
   return tile_dispatch(a, grid, st);
 }
-hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st) {
+hipError_t launch_tile_f32(const KArgs<float> &a, int grid, hipStream_t st) {
   return tile_dispatch(a, grid, st);
+}
+
+hipError_t launch_finish(const double *partial, int nlong, const int32_t *lsp, const int32_t *lseg,
+                         const int32_t *rows, bool assign, const int32_t *crow, const double *cval, int64_t nslots,
+                         double alpha, double beta, double *y, hipStream_t st) {
+  const int64_t g = nlong + (nslots + kBlock - 1) / kBlock;
+  if (g <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finish, dim3((unsigned)g), dim3(kBlock), 0, st, partial, nlong, lsp, lseg, rows, assign ? 1 : 0,
+                     crow, cval, nslots, alpha, beta, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_fix(const int32_t *crow, const double *cval, int64_t nslots, double alpha, double beta,
+                           double *y, hipStream_t st) {
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_slot_fix, dim3(nblk(nslots)), dim3(kBlock), 0, st, crow, cval, nslots, alpha, beta, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_unit_window(const int32_t *col, int32_t *f, int stride, int off, int64_t nunits, hipStream_t st) {
+  if (nunits <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unit_window, dim3((unsigned)nunits), dim3(kBlock), 0, st, col, f, stride, off, nunits);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_units(const Coord *coords, int64_t units, MUnit *out, hipStream_t st) {
+  if (units <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_merge_units, dim3(nblk(units)), dim3(kBlock), 0, st, coords, units, out);
+  return hipGetLastError();
 }
 
 }  // namespace spx

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/spx_capi.h"
+#include "spx_build.h"
 #include "spx_internal.h"
 
 using namespace spx;
@@ -49,12 +50,14 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 int default_lanes(double mean) {
-  // lanes per row for the CSR-vector kernel from the mean row length
-  if (mean <= 2) return 2;
-  if (mean <= 4) return 4;
-  if (mean <= 8) return 8;
-  if (mean <= 16) return 16;
-  if (mean <= 32) return 32;
+  // lanes per row for the CSR-vector kernel: each lane takes two nonzeros
+  // per step, so L lanes cover a row of up to 2L - 1 nonzeros (at either
+  // alignment) in one step
+  if (mean <= 3) return 2;
+  if (mean <= 7) return 4;
+  if (mean <= 15) return 8;
+  if (mean <= 31) return 16;
+  if (mean <= 63) return 32;
   return 64;
 }
 
@@ -80,7 +83,8 @@ struct spx_plan {
   int32_t *d_col = nullptr;      // CSR columns, or tile layout for kernel 3
   double *d_val = nullptr;       // FP64 values (CSR or tile layout)
   float *d_val32 = nullptr;      // FP32 values (precision form)
-  XPair *d_xhl = nullptr;        // precision form: split x
+  VUnit *d_vunits = nullptr;     // kernel 1: unit table
+  MUnit *d_munits = nullptr;     // kernel 2: unit table
   Part *d_parts = nullptr;
   std::vector<Part> parts;
   Unit *d_units = nullptr;
@@ -92,8 +96,7 @@ struct spx_plan {
   int32_t *d_crow = nullptr;
   double *d_cval = nullptr;
   int64_t ncarry = 0;
-  Run *d_runs = nullptr;
-  int64_t nruns = 0;
+  TUnit *d_tunits = nullptr;     // kernel 3: unit table
   int32_t *d_empty = nullptr;
   int64_t nempty = 0;
   int *d_queue = nullptr;
@@ -137,11 +140,12 @@ struct spx_plan {
   std::vector<hipEvent_t> done;
   hipEvent_t fork = nullptr;
   int64_t bytes = 0;
+  double upload_ms = 0.0, build_ms = 0.0;  // host time: copies of the input, the whole build
 
   ~spx_plan() {
-    for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_xhl,
+    for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_vunits, (void *)d_munits,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
-                    (void *)d_crow, (void *)d_cval, (void *)d_runs, (void *)d_empty, (void *)d_queue,
+                    (void *)d_crow, (void *)d_cval, (void *)d_tunits, (void *)d_empty, (void *)d_queue,
                     (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_xr, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
                     (void *)d_scol12, (void *)d_scol32, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
@@ -488,6 +492,11 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   return SPX_OK;
 }
 
+static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
+                     const std::vector<int32_t> &long_rows);
+static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
+                     const std::vector<int32_t> &long_rows);
+
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int m = d->m, n = d->n;
   const int64_t nnz = d->nnz;
@@ -552,20 +561,31 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   }
   P->nlong = (int)long_rows.size();
 
+  if (d->kernel == SPX_KERNEL_SEG) return build_seg(P, d, rp, long_rows);
+  return build_csr(P, d, rp, long_rows);
+}
+
+// Kernel 4: the unit table is built on the host (it needs each unit's column
+// window), then uploaded in the segmented layout.
+static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
+                     const std::vector<int32_t> &long_rows) {
+  const int m = d->m, n = d->n;
+  const int64_t nnz = d->nnz;
+  const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
   const int32_t *src_col = d->col;
   const double *src_val = d->val;
-  bool src_dev = on_dev;
   std::vector<int32_t> hcol, rcol, lcol, lrp;
   std::vector<double> hval, rval, lval;
-  if (on_dev && (P->nlong > 0 || d->kernel == SPX_KERNEL_SEG)) {
+  if (on_dev) {
     hcol.resize(nnz);
     hval.resize(nnz);
     HIP_TRY(hipMemcpy(hcol.data(), d->col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(hval.data(), d->val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
     src_col = hcol.data();
     src_val = hval.data();
-    src_dev = false;
   }
+  for (int64_t k = 0; k < nnz; k++)
+    if (src_col[k] < 0 || src_col[k] >= n) return fail(SPX_EARG, "column index out of range");
   if (P->nlong > 0) {
     std::vector<char> is_long(m, 0);
     for (int r : long_rows) is_long[r] = 1;
@@ -610,7 +630,6 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     rp.swap(rrp);
     src_col = rcol.data();
     src_val = rval.data();
-    src_dev = false;
   }
   const int64_t nnzr = rp[m];
   P->nnz_reg = nnzr;
@@ -636,8 +655,8 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     if ((rc = P->alloc(&P->d_partial, P->nlseg))) return rc;
   }
 
-  // ---- kernel 4: the regular rows join the long slices in one launch ----
-  if (d->kernel == SPX_KERNEL_SEG) {
+  // ---- the regular rows join the long slices in one launch ----
+  {
     P->items = (int)(seg_target / (kSegBlock * 8));
     // empty rows of A (the long rows are written by the reduction)
     std::vector<char> is_long(m, 0);
@@ -667,18 +686,291 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     }
     P->nempty = (int64_t)empty.size();
     if ((rc = P->upload(&P->d_empty, empty.data(), P->nempty))) return rc;
-    if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
     HIP_TRY(hipDeviceSynchronize());
     return SPX_OK;
   }
-  if (P->nlong > 0) {
-    int rc;
-    if ((rc = seg_upload(P, H))) return rc;
-    HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&P->ldone, hipEventDisableTiming));
+}
+
+// Device memory for the duration of a plan build (freed on every path).
+template <typename T>
+struct DevTmp {
+  T *p = nullptr;
+  DevTmp() = default;
+  DevTmp(const DevTmp &) = delete;
+  DevTmp &operator=(const DevTmp &) = delete;
+  ~DevTmp() {
+    if (p) (void)hipFree(p);
   }
-  const hipMemcpyKind h2d = src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  int alloc(int64_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)(count > 0 ? count : 1));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return SPX_OK;
+  }
+};
+
+// The long rows' (column block, row) slices in the segmented engine's
+// layout, built on the device from the long rows' compact arrays (columns
+// sorted within each row).  Slices are ordered by block, then row; a block
+// is one unit unless it holds more than `target` nonzeros, then it is cut
+// into nearly equal units of whole slices.  Every unit stages its block of
+// x.  Slice k sums into partial[k]; lsp/lseg list each row's slices in
+// column order for k_long_reduce.
+static int build_long_slices(spx_plan *P, int n, int32_t *lcol, double *lval, const std::vector<int32_t> &lrp,
+                             int64_t target) {
+  const int nlong = P->nlong;
+  const int nb = P->nblocks;
+  int rc;
+  DevTmp<int32_t> d_lrp, d_cut, d_spos, d_ssrc;
+  DevTmp<int> d_flag;
+  if ((rc = d_lrp.alloc(nlong + 1)) || (rc = d_flag.alloc(nlong))) return rc;
+  HIP_TRY(hipMemcpy(d_lrp.p, lrp.data(), sizeof(int32_t) * (nlong + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(d_flag.p, 0, sizeof(int) * nlong));
+  HIP_TRY(launch_long_unsorted(lcol, d_lrp.p, nlong, d_flag.p, 0));
+  std::vector<int> unsorted(nlong);
+  HIP_TRY(hipMemcpy(unsorted.data(), d_flag.p, sizeof(int) * nlong, hipMemcpyDeviceToHost));
+  for (int r = 0; r < nlong; r++) {
+    if (!unsorted[r]) continue;
+    // rare: a long row with unsorted columns is sorted on the host (stable,
+    // so equal columns keep their order)
+    const int64_t b = lrp[r], len = lrp[r + 1] - lrp[r];
+    std::vector<int32_t> c(len);
+    std::vector<double> v(len);
+    HIP_TRY(hipMemcpy(c.data(), lcol + b, sizeof(int32_t) * len, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(v.data(), lval + b, sizeof(double) * len, hipMemcpyDeviceToHost));
+    std::vector<int64_t> ord(len);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return c[x] < c[y]; });
+    std::vector<int32_t> c2(len);
+    std::vector<double> v2(len);
+    for (int64_t k = 0; k < len; k++) {
+      c2[k] = c[ord[k]];
+      v2[k] = v[ord[k]];
+    }
+    HIP_TRY(hipMemcpy(lcol + b, c2.data(), sizeof(int32_t) * len, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(lval + b, v2.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+  }
+  const int64_t per = (int64_t)nb + 1;
+  if ((rc = d_cut.alloc(per * nlong))) return rc;
+  HIP_TRY(launch_long_cuts(lcol, d_lrp.p, nlong, nb, kColBlock, d_cut.p, 0));
+  std::vector<int32_t> cut((size_t)(per * nlong));
+  HIP_TRY(hipMemcpy(cut.data(), d_cut.p, sizeof(int32_t) * cut.size(), hipMemcpyDeviceToHost));
+
+  // slices, block-major
+  std::vector<int32_t> sb, slen, srow;
+  std::vector<int64_t> bstart(nb + 1, 0);  // slices of block b: [bstart[b], bstart[b + 1])
+  for (int b = 0; b < nb; b++) {
+    bstart[b] = (int64_t)sb.size();
+    for (int r = 0; r < nlong; r++) {
+      const int32_t s0 = cut[(size_t)(r * per + b)], s1 = cut[(size_t)(r * per + b + 1)];
+      if (s1 <= s0) continue;
+      sb.push_back(s0);
+      slen.push_back(s1 - s0);
+      srow.push_back(r);
+    }
+  }
+  bstart[nb] = (int64_t)sb.size();
+  const int64_t ns = (int64_t)sb.size();
+  P->nlseg = ns;
+  std::vector<int32_t> lsp(nlong + 1, 0), lseg((size_t)ns);
+  for (int32_t r : srow) lsp[r + 1]++;
+  for (int r = 0; r < nlong; r++) lsp[r + 1] += lsp[r];
+  {
+    std::vector<int32_t> fill(lsp.begin(), lsp.end() - 1);
+    for (int64_t k = 0; k < ns; k++) lseg[fill[srow[k]]++] = (int32_t)k;  // column order per row
+  }
+  // units
+  std::vector<SegUnit> units;
+  std::vector<int32_t> spos((size_t)ns);
+  int64_t pos = 0;
+  for (int b = 0; b < nb; b++) {
+    const int64_t i0 = bstart[b], i1 = bstart[b + 1];
+    int64_t tot = 0;
+    for (int64_t k = i0; k < i1; k++) tot += slen[k];
+    if (tot == 0) continue;
+    const int64_t parts = (tot + target - 1) / target;
+    const int64_t tb = parts > 1 ? (tot * 11 + parts * 10 - 1) / (parts * 10) : target;
+    const int32_t cmin = b * kColBlock, cwin = std::min<int32_t>(kColBlock, n - cmin);
+    for (int64_t k = i0; k < i1;) {
+      int64_t el = 0, q = k;
+      while (q < i1 && (q == k || el + slen[q] <= tb)) el += slen[q++];
+      const int64_t E0 = (pos + 7) & ~(int64_t)7;
+      if (E0 + el + 16 >= INT32_MAX) return fail(SPX_ERANGE, "padded long-row nonzeros must stay below 2^31");
+      units.push_back(SegUnit{(int32_t)E0, (int32_t)(E0 + el), (int32_t)k, (int32_t)(q - k), (int32_t)k, cmin, cwin,
+                              SEG_STAGE | SEG_PARTIAL});
+      int64_t at = E0;
+      for (int64_t t = k; t < q; t++) {
+        spos[t] = (int32_t)at;
+        at += slen[t];
+      }
+      pos = E0 + el;
+      k = q;
+    }
+  }
+  const int64_t nu = (int64_t)units.size();
+  const int64_t total = ((pos + 7) & ~(int64_t)7) + 8, groups = total / 8;
+  P->nsunits = P->nslong = P->nstaged_units = nu;
+  P->nsstaged = 0;
+  P->seg_split = false;
+  P->sstage = nu > 0;
+  // XCD ranges of about equal work (nonzeros, staged x entries, a fixed cost per unit)
+  std::vector<double> wsum((size_t)nu + 1, 0.0);
+  for (int64_t u = 0; u < nu; u++)
+    wsum[u + 1] = wsum[u] + (double)(units[u].e1 - units[u].e0) + 0.5 * units[u].cwin + 256.0;
+  std::vector<int32_t> xr(3 * (kXcd + 1), 0);
+  xr[0] = 0;
+  for (int x = 1; x < kXcd; x++)
+    xr[x] = (int32_t)(std::lower_bound(wsum.begin(), wsum.end() - 1, wsum[nu] * x / kXcd) - wsum.begin());
+  xr[kXcd] = (int32_t)nu;
+  for (int x = 1; x <= kXcd; x++) xr[x] = std::max(xr[x], xr[x - 1]);
+  for (int x = 0; x <= kXcd; x++) xr[2 * (kXcd + 1) + x] = xr[x];  // "unstaged" class: the same units
+  if ((rc = P->upload(&P->d_sunits, units.data(), nu))) return rc;
+  if ((rc = P->upload(&P->d_sxr, xr.data(), 3 * (kXcd + 1)))) return rc;
+  if ((rc = d_spos.alloc(ns)) || (rc = d_ssrc.alloc(ns))) return rc;
+  if (ns) {
+    HIP_TRY(hipMemcpy(d_spos.p, spos.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_ssrc.p, sb.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+  }
+  if ((rc = P->alloc(&P->d_sstart, groups)) || (rc = P->alloc(&P->d_scol12, 3 * groups))) return rc;
+  const bool f32 = P->precision == SPX_PREC_F32;
+  if (f32) {
+    if ((rc = P->alloc(&P->d_sval32, total))) return rc;
+    HIP_TRY(hipMemset(P->d_sval32, 0, sizeof(float) * total));
+  } else {
+    if ((rc = P->alloc(&P->d_sval, total))) return rc;
+    HIP_TRY(hipMemset(P->d_sval, 0, sizeof(double) * total));
+  }
+  HIP_TRY(launch_seg_fill(P->d_sunits, nu, d_spos.p, d_ssrc.p, lcol, lval, groups, P->d_sstart, P->d_scol12,
+                          f32 ? nullptr : P->d_sval, f32 ? P->d_sval32 : nullptr, 0));
+  if ((rc = P->upload(&P->d_lsp, lsp.data(), nlong + 1))) return rc;
+  if ((rc = P->upload(&P->d_lseg, lseg.data(), ns))) return rc;
+  if ((rc = P->alloc(&P->d_partial, ns))) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  return SPX_OK;
+}
+
+// Kernels 1-3: the input reaches device memory in one copy per array and
+// the conversion runs on the device (long rows split off and laid out for
+// the segmented engine, columns checked), so a one-shot call costs the
+// upload and a few passes at HBM speed; the host touches only the row
+// pointer.
+static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
+                     const std::vector<int32_t> &long_rows) {
+  const int m = d->m, n = d->n;
+  const int64_t nnz = d->nnz;
+  const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
+  const hipMemcpyKind in_kind = on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  // FP32 values in CSR order (kernel 3 converts while it builds its tiles)
+  const bool f32_csr = P->precision == SPX_PREC_F32 && d->kernel != SPX_KERNEL_TILE;
+  int rc;
+
+  // long rows: their input ranges in row order, and the regular row pointer
+  std::vector<LongRange> lr((size_t)P->nlong);
+  std::vector<int32_t> lrp((size_t)P->nlong + 1, 0);
+  int64_t ln = 0;
+  for (int j = 0; j < P->nlong; j++) {
+    const int r = long_rows[j];
+    lr[j] = LongRange{rp[r], rp[r + 1], ln};
+    ln += rp[r + 1] - rp[r];
+    lrp[j + 1] = (int32_t)ln;
+  }
+  P->long_nnz = ln;
+  const int64_t nnzr = nnz - ln;
+  P->nnz_reg = nnzr;
+  if (P->nlong > 0) {
+    std::vector<int32_t> rrp((size_t)m + 1, 0);
+    size_t j = 0;
+    int64_t sub = 0;
+    for (int i = 0; i < m; i++) {
+      if (j < long_rows.size() && long_rows[j] == i) {
+        sub += rp[i + 1] - rp[i];
+        j++;
+      }
+      rrp[i + 1] = (int32_t)(rp[i + 1] - sub);
+    }
+    rp.swap(rrp);
+  }
+
+  // ---- the regular CSR on the device, padded by two zero entries ----
+  DevTmp<int> d_bad;
+  if ((rc = d_bad.alloc(1))) return rc;
+  HIP_TRY(hipMemset(d_bad.p, 0, sizeof(int)));
+  if ((rc = P->alloc(&P->d_col, nnzr + 2))) return rc;
+  HIP_TRY(hipMemset(P->d_col + nnzr, 0, 2 * sizeof(int32_t)));
+  if (f32_csr) {
+    if ((rc = P->alloc(&P->d_val32, nnzr + 2))) return rc;
+    HIP_TRY(hipMemset(P->d_val32 + nnzr, 0, 2 * sizeof(float)));
+  } else {
+    if ((rc = P->alloc(&P->d_val, nnzr + 2))) return rc;
+    HIP_TRY(hipMemset(P->d_val + nnzr, 0, 2 * sizeof(double)));
+  }
+  DevTmp<int32_t> craw, lcol;
+  DevTmp<double> vraw, lval;
+  const auto t_up = std::chrono::steady_clock::now();
+  if (P->nlong == 0) {
+    if (nnz > 0) {
+      HIP_TRY(hipMemcpy(P->d_col, d->col, sizeof(int32_t) * nnz, in_kind));
+      if (f32_csr) {
+        const double *v = d->val;
+        if (!on_dev) {
+          if ((rc = vraw.alloc(nnz))) return rc;
+          HIP_TRY(hipMemcpy(vraw.p, d->val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+          v = vraw.p;
+        }
+        HIP_TRY(launch_to_f32(v, P->d_val32, nnz, 0));
+      } else {
+        HIP_TRY(hipMemcpy(P->d_val, d->val, sizeof(double) * nnz, in_kind));
+      }
+    }
+    P->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
+    HIP_TRY(launch_check_cols(P->d_col, nnz, n, d_bad.p, 0));
+  } else {
+    const int32_t *col = d->col;
+    const double *val = d->val;
+    if (!on_dev) {
+      if ((rc = craw.alloc(nnz)) || (rc = vraw.alloc(nnz))) return rc;
+      HIP_TRY(hipMemcpy(craw.p, d->col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(vraw.p, d->val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+      col = craw.p;
+      val = vraw.p;
+    }
+    P->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
+    HIP_TRY(launch_check_cols(col, nnz, n, d_bad.p, 0));
+    DevTmp<LongRange> d_lr;
+    if ((rc = d_lr.alloc(P->nlong)) || (rc = lcol.alloc(ln)) || (rc = lval.alloc(ln))) return rc;
+    HIP_TRY(hipMemcpy(d_lr.p, lr.data(), sizeof(LongRange) * lr.size(), hipMemcpyHostToDevice));
+    HIP_TRY(launch_split_long(col, val, nnz, d_lr.p, P->nlong, P->d_col, f32_csr ? nullptr : P->d_val,
+                              f32_csr ? P->d_val32 : nullptr, lcol.p, lval.p, 0));
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  {
+    int bad = 0;
+    HIP_TRY(hipMemcpy(&bad, d_bad.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) return fail(SPX_EARG, "column index out of range");
+  }
+  craw.reset();
+  vraw.reset();
+
+  // ---- long rows: (column block, row) slices for the segmented engine ----
+  if (P->nlong > 0) {
+    P->nblocks = (int)(((int64_t)n + kColBlock - 1) / kColBlock);
+    if ((int64_t)P->nblocks * P->nlong >= INT32_MAX) return fail(SPX_ERANGE, "too many long-row blocks");
+    if (P->long_nnz + 8 * (int64_t)P->nblocks * P->nlong + 64 >= INT32_MAX)
+      return fail(SPX_ERANGE, "padded long-row nonzeros must stay below 2^31 per plan");
+    if ((rc = build_long_slices(P, n, lcol.p, lval.p, lrp, std::max<int64_t>((int64_t)kSegBlock * 8 * 2, kLongUnit))))
+      return rc;
+    if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
+  }
+  lcol.reset();
+  lval.reset();
 
   // ---- partitions of the regular part ----
   int kind = PART_NNZ, np = 1;
@@ -697,7 +989,6 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     np = (int)T;
   }
 
-  int rc;
   if ((rc = P->upload(&P->d_rowptr, rp.data(), m + 1))) return rc;
   if ((rc = P->alloc(&P->d_parts, np))) return rc;
   if (nnzr == 0) {
@@ -728,25 +1019,25 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     P->sigma = d->sigma > 0 ? d->sigma : tile_sigma_auto(m, nnzr);
     if (P->sigma < 1 || P->sigma > 32) return fail(SPX_EARG, "sigma must be in 1..32");
   }
+  // units never cross partitions; every unit u (global numbering, in
+  // nonzero order) owns carry slots 2u and 2u + 1
   const int tw = kBlock * P->items;
-  int64_t uoff = 0, coff = 0, cooff = 0, toff = 0, tpoff = 0;
+  const int64_t tiles_per_unit = 4 * kTileUnitChunks;
+  int64_t uoff = 0, cooff = 0, toff = 0, tpoff = 0;
   for (auto &q : P->parts) {
     const int64_t rows = (int64_t)q.row_hi - q.row_lo;
     const int64_t nzq = q.e - q.s + 1;
     q.unit_off = uoff;
-    q.carry_off = coff;
+    q.carry_off = 2 * uoff;
+    q.cslot1 = -1;
     if (P->kernel == SPX_KERNEL_VECTOR) {
-      const int G = kBlock / P->lanes;
+      const int64_t G = (int64_t)(kBlock / P->lanes) * kVecRows(P->lanes);
       q.units = (rows + G - 1) / G;
-      q.cslot1 = coff + 1;
-      coff += 2;
     } else if (P->kernel == SPX_KERNEL_MERGE) {
       const int64_t L = (rows - q.eflag) + nzq;
       q.units = (L + tw - 1) / tw;
       q.coord_off = cooff;
       cooff += q.units + 1;
-      q.cslot1 = coff + 1 + 2 * q.units;
-      coff += 2 + 2 * q.units;
     } else {
       const int64_t w = (int64_t)kOmega * P->sigma;
       q.ntiles = (nzq + w - 1) / w;
@@ -754,23 +1045,63 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
       q.tp_off = tpoff;
       toff += q.ntiles;
       tpoff += q.ntiles + 1;
-      q.units = (q.ntiles + 3) / 4;
-      q.cslot1 = coff + 1 + 2 * q.ntiles;
-      coff += 2 + 2 * q.ntiles;
+      q.units = (q.ntiles + tiles_per_unit - 1) / tiles_per_unit;
     }
     uoff += q.units;
   }
   P->nunits = uoff;
-  P->ncarry = coff;
+  P->ncarry = 2 * uoff;
   P->ncoords = cooff;
   P->ntiles = toff;
   P->ntp = tpoff;
   HIP_TRY(hipMemcpy(P->d_parts, P->parts.data(), sizeof(Part) * P->parts.size(), hipMemcpyHostToDevice));
 
-  std::vector<Unit> units(std::max<int64_t>(1, P->nunits));
-  for (size_t pi = 0; pi < P->parts.size(); pi++)
-    for (int64_t u = 0; u < P->parts[pi].units; u++) units[P->parts[pi].unit_off + u] = Unit{(int32_t)pi, (int32_t)u};
-  if ((rc = P->upload(&P->d_units, units.data(), P->nunits))) return rc;
+  if (P->kernel == SPX_KERNEL_VECTOR) {
+    // rows of each unit and the partition's nonzero range; the window fields
+    // first carry the unit's nonzero range
+    const int64_t G = (int64_t)(kBlock / P->lanes) * kVecRows(P->lanes);
+    std::vector<VUnit> vu(std::max<int64_t>(1, P->nunits));
+    for (const Part &q : P->parts)
+      for (int64_t u = 0; u < q.units; u++) {
+        VUnit &v = vu[q.unit_off + u];
+        v.r_lo = (int32_t)(q.row_lo + u * G);
+        v.r_hi = (int32_t)std::min<int64_t>(q.row_hi, v.r_lo + G);
+        v.s = (int32_t)q.s;
+        v.e1 = (int32_t)(q.e + 1);
+        v.cmin = std::max<int32_t>(rp[v.r_lo], v.s);
+        v.cwin = std::min<int32_t>(rp[v.r_hi], v.e1);
+        v.pad0 = v.pad1 = 0;
+      }
+    if ((rc = P->upload(&P->d_vunits, vu.data(), P->nunits))) return rc;
+    HIP_TRY(launch_unit_window(P->d_col, reinterpret_cast<int32_t *>(P->d_vunits), 8, 4, P->nunits, 0));
+  }
+  if (P->kernel == SPX_KERNEL_TILE) {
+    // units of tiles_per_unit tiles; the window fields first carry the
+    // unit's nonzero range in the CSR arrays
+    std::vector<TUnit> tu(std::max<int64_t>(1, P->nunits));
+    const int64_t w = (int64_t)kOmega * P->sigma;
+    for (const Part &q : P->parts) {
+      const int64_t nzq = q.e - q.s + 1;
+      for (int64_t u = 0; u < q.units; u++) {
+        TUnit &t = tu[q.unit_off + u];
+        const int64_t lt0 = u * tiles_per_unit;
+        t.gt0 = (int32_t)(q.tile_off + lt0);
+        t.nt = (int32_t)std::min<int64_t>(tiles_per_unit, q.ntiles - lt0);
+        t.lt0 = (int32_t)lt0;
+        t.tp0 = (int32_t)(q.tp_off + lt0);
+        t.nz = (int32_t)nzq;
+        t.row_lo = q.row_lo;
+        t.s = (int32_t)q.s;
+        t.gu = (int32_t)(q.unit_off + u);
+        t.cmin = (int32_t)(q.s + lt0 * w);
+        t.cwin = (int32_t)(q.s + std::min<int64_t>((lt0 + t.nt) * w, nzq));
+        t.flags = (q.sflag ? kTuSplitHead : 0) | (q.eflag ? kTuSplitTail : 0);
+        t.last_lr = q.row_hi - q.row_lo - 1;
+      }
+    }
+    if ((rc = P->upload(&P->d_tunits, tu.data(), P->nunits))) return rc;
+    HIP_TRY(launch_unit_window(P->d_col, reinterpret_cast<int32_t *>(P->d_tunits), 12, 8, P->nunits, 0));
+  }
 
   if ((rc = P->alloc(&P->d_crow, P->ncarry))) return rc;
   if ((rc = P->alloc(&P->d_cval, P->ncarry))) return rc;
@@ -779,66 +1110,46 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
 
   // ---- regular payload ----
   if (P->kernel == SPX_KERNEL_TILE) {
+    // the CSR arrays become the tile layout (64 lanes x sigma per tile)
     const int64_t tot = P->ntiles * kOmega * P->sigma;
-    int32_t *col_tmp = nullptr;
-    double *val_tmp = nullptr;
-    if (nnzr > 0) {
-      if (hipMalloc((void **)&col_tmp, sizeof(int32_t) * nnzr) != hipSuccess ||
-          hipMalloc((void **)&val_tmp, sizeof(double) * nnzr) != hipSuccess) {
-        if (col_tmp) (void)hipFree(col_tmp);
-        return fail(SPX_EMEM, "hipMalloc: staging for tile conversion");
-      }
-      hipError_t e = hipMemcpy(col_tmp, src_col, sizeof(int32_t) * nnzr, h2d);
-      if (e == hipSuccess) e = hipMemcpy(val_tmp, src_val, sizeof(double) * nnzr, h2d);
-      if (e != hipSuccess) {
-        (void)hipFree(col_tmp);
-        (void)hipFree(val_tmp);
-        HIP_TRY(e);
-      }
-    }
+    int32_t *csr_col = P->d_col;
+    double *csr_val = P->d_val;
+    P->d_col = nullptr;
+    P->d_val = nullptr;
     rc = P->alloc(&P->d_tp, P->ntp);
-    if (!rc) rc = P->alloc(&P->d_tf, P->ntiles * kOmega);
+    if (!rc) rc = P->alloc(&P->d_tf, (P->ntiles + 1) * kOmega);  // one tile of padding: a lane reads its right neighbour's word
+    if (!rc) rc = hipMemset(P->d_tf + P->ntiles * kOmega, 0, sizeof(uint32_t) * kOmega) == hipSuccess ? SPX_OK : SPX_EDEV;
     if (!rc) rc = P->alloc(&P->d_col, tot);
     if (!rc) rc = P->precision == SPX_PREC_F32 ? P->alloc(&P->d_val32, tot) : P->alloc(&P->d_val, tot);
     hipError_t e = hipSuccess;
     if (!rc && nnzr > 0)
-      e = launch_tile_convert(P->d_rowptr, col_tmp, val_tmp, P->parts.data(), P->d_parts, (int)P->parts.size(),
+      e = launch_tile_convert(P->d_rowptr, csr_col, csr_val, P->parts.data(), P->d_parts, (int)P->parts.size(),
                               P->sigma, P->d_tp, P->d_tf, P->d_col, P->d_val, P->d_val32, 0);
     if (!rc && e == hipSuccess) e = hipDeviceSynchronize();
-    if (col_tmp) (void)hipFree(col_tmp);
-    if (val_tmp) (void)hipFree(val_tmp);
+    (void)hipFree(csr_col);
+    (void)hipFree(csr_val);
+    P->bytes -= (int64_t)(nnzr + 2) * (sizeof(int32_t) + sizeof(double));
     if (rc) return rc;
     HIP_TRY(e);
-    // empty rows (long rows included) are written by their own small kernel
+    // empty rows are written by their own small kernel (the long rows, empty
+    // here, by the long-row fold)
     std::vector<int32_t> empty;
-    for (int i = 0; i < m; i++)
-      if (rp[i] == rp[i + 1]) empty.push_back(i);
+    size_t lj = 0;
+    for (int i = 0; i < m; i++) {
+      const bool is_long = lj < long_rows.size() && long_rows[lj] == i;
+      if (is_long) lj++;
+      if (rp[i] == rp[i + 1] && !is_long) empty.push_back(i);
+    }
     P->nempty = (int64_t)empty.size();
     if ((rc = P->upload(&P->d_empty, empty.data(), P->nempty))) return rc;
-  } else {
-    if ((rc = P->alloc(&P->d_col, nnzr))) return rc;
-    if (nnzr) HIP_TRY(hipMemcpy(P->d_col, src_col, sizeof(int32_t) * nnzr, h2d));
-    if (P->precision == SPX_PREC_F32) {
-      double *tmp = nullptr;
-      if ((rc = P->alloc(&P->d_val32, nnzr))) return rc;
-      if (nnzr) {
-        HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * nnzr));
-        hipError_t e = hipMemcpy(tmp, src_val, sizeof(double) * nnzr, h2d);
-        if (e == hipSuccess) e = launch_to_f32(tmp, P->d_val32, nnzr, 0);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-        (void)hipFree(tmp);
-        HIP_TRY(e);
-      }
-    } else {
-      if ((rc = P->alloc(&P->d_val, nnzr))) return rc;
-      if (nnzr) HIP_TRY(hipMemcpy(P->d_val, src_val, sizeof(double) * nnzr, h2d));
-    }
-    if (P->kernel == SPX_KERNEL_MERGE) {
-      if ((rc = P->alloc(&P->d_coords, P->ncoords))) return rc;
-      HIP_TRY(launch_merge_coords(P->d_rowptr, P->d_parts, (int)P->parts.size(), P->ncoords, tw, P->d_coords, 0));
-    }
+  } else if (P->kernel == SPX_KERNEL_MERGE) {
+    if ((rc = P->alloc(&P->d_coords, P->ncoords))) return rc;
+    HIP_TRY(launch_merge_coords(P->d_rowptr, P->d_parts, (int)P->parts.size(), P->ncoords, tw, P->d_coords, 0));
+    if ((rc = P->alloc(&P->d_munits, P->nunits))) return rc;
+    for (const Part &q : P->parts)
+      HIP_TRY(launch_merge_units(P->d_coords + q.coord_off, q.units, P->d_munits + q.unit_off, 0));
+    HIP_TRY(launch_unit_window(P->d_col, reinterpret_cast<int32_t *>(P->d_munits), 8, 4, P->nunits, 0));
   }
-  if (P->precision == SPX_PREC_F32 && (rc = P->alloc(&P->d_xhl, n))) return rc;
 
   // ---- launch layout: partitions -> streams ----
   int nstreams = 1;
@@ -880,33 +1191,6 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     HIP_TRY(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
   }
 
-  // ---- carry runs from a dry execute ----
-  if (P->ncarry > 0 && nnzr > 0) {
-    double *x0 = nullptr, *y0 = nullptr;
-    HIP_TRY(hipMalloc((void **)&x0, sizeof(double) * std::max(n, 1)));
-    hipError_t e = hipMalloc((void **)&y0, sizeof(double) * m);
-    if (e == hipSuccess) e = hipMemset(x0, 0, sizeof(double) * std::max(n, 1));
-    int erc = e == hipSuccess ? execute_impl(P, 1.0, x0, 0.0, y0, 0, nullptr, nullptr, true) : SPX_EDEV;
-    if (erc == SPX_OK) e = hipDeviceSynchronize();
-    std::vector<int32_t> crow(P->ncarry);
-    if (erc == SPX_OK && e == hipSuccess)
-      e = hipMemcpy(crow.data(), P->d_crow, sizeof(int32_t) * P->ncarry, hipMemcpyDeviceToHost);
-    (void)hipFree(x0);
-    if (y0) (void)hipFree(y0);
-    if (erc != SPX_OK) return erc;
-    HIP_TRY(e);
-    std::vector<Run> runs;
-    for (int64_t k = 0; k < P->ncarry; k++) {
-      const int32_t r = crow[k];
-      if (r < 0) continue;
-      if (!runs.empty() && runs.back().row == r)
-        runs.back().end = (int32_t)(k + 1);
-      else
-        runs.push_back(Run{r, (int32_t)k, (int32_t)(k + 1)});
-    }
-    P->nruns = (int64_t)runs.size();
-    if ((rc = P->upload(&P->d_runs, runs.data(), P->nruns))) return rc;
-  }
   HIP_TRY(hipDeviceSynchronize());
   return SPX_OK;
 }
@@ -917,7 +1201,9 @@ int spx_plan_create(spx_plan_t *out, const spx_plan_desc *d) {
   if (!out || !d) return fail(SPX_EARG, "null argument");
   *out = nullptr;
   std::unique_ptr<spx_plan> P(new spx_plan());
+  const auto t0 = std::chrono::steady_clock::now();
   int rc = plan_build(P.get(), d);
+  P->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (rc != SPX_OK) return rc;
   *out = P.release();
   return SPX_OK;
@@ -930,9 +1216,9 @@ int spx_plan_destroy(spx_plan_t p) {
 
 }  // extern "C"
 
-template <typename V, typename X>
-static hipError_t launch_main(spx_plan *P, const KArgs<V, X> &base, const spx_plan::Launch &L, hipStream_t st) {
-  KArgs<V, X> a = base;
+template <typename V>
+static hipError_t launch_main(spx_plan *P, const KArgs<V> &base, const spx_plan::Launch &L, hipStream_t st) {
+  KArgs<V> a = base;
   a.unit_begin = L.ubeg;
   a.unit_count = L.ucnt;
   a.queue = nullptr;
@@ -952,8 +1238,8 @@ static hipError_t launch_main(spx_plan *P, const KArgs<V, X> &base, const spx_pl
   }
 }
 
-template <typename V, typename X>
-static hipError_t run_regular(spx_plan *P, const KArgs<V, X> &a, hipStream_t st) {
+template <typename V>
+static hipError_t run_regular(spx_plan *P, const KArgs<V> &a, hipStream_t st) {
   hipError_t e = hipSuccess;
   if (P->streams.empty()) {
     for (auto &L : P->launches)
@@ -978,9 +1264,9 @@ static hipError_t run_regular(spx_plan *P, const KArgs<V, X> &a, hipStream_t st)
   return e;
 }
 
-template <typename V, typename X>
-static KArgs<V, X> make_args(spx_plan *P, const V *val, const X *x, double alpha, double beta, double *y) {
-  KArgs<V, X> a = {};
+template <typename V>
+static KArgs<V> make_args(spx_plan *P, const V *val, const double *x, double alpha, double beta, double *y) {
+  KArgs<V> a = {};
   a.parts = P->d_parts;
   a.units = P->d_units;
   a.rowptr = P->d_rowptr;
@@ -992,7 +1278,9 @@ static KArgs<V, X> make_args(spx_plan *P, const V *val, const X *x, double alpha
   a.y = y;
   a.crow = P->d_crow;
   a.cval = P->d_cval;
-  a.coords = P->d_coords;
+  a.vunits = P->d_vunits;
+  a.munits = P->d_munits;
+  a.tunits = P->d_tunits;
   a.tile_ptr = P->d_tp;
   a.tile_flags = P->d_tf;
   a.sigma = P->sigma;
@@ -1009,8 +1297,8 @@ static hipError_t launch_seg(spx_plan *P, int cls, const double *x, double alpha
   const int32_t *xr = P->d_sxr + cls * (kXcd + 1);
   const bool stage = cls == SEG_STAGED || (cls == SEG_ALL && P->sstage);
   if (P->precision == SPX_PREC_F32) {
-    SegArgs<float, XPair> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval32,
-                               P->d_xhl, alpha, beta, y, P->d_partial};
+    SegArgs<float, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval32,
+                                x, alpha, beta, y, P->d_partial};
     return launch_seg_f32(a, stage, st);
   }
   SegArgs<double, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval,
@@ -1027,8 +1315,11 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
   if (!P) return fail(SPX_EARG, "null plan");
   if (P->m == 0) return SPX_OK;
   if (!y || (P->n > 0 && !x)) return fail(SPX_EARG, "null x or y");
+  if (P->nnz == 0) {  // no nonzeros (x may be empty): y = beta * y
+    HIP_TRY(launch_scale_y(beta, y, P->m, st));
+    return SPX_OK;
+  }
   const bool f32 = P->precision == SPX_PREC_F32;
-  if (f32) HIP_TRY(launch_split_x(x, P->d_xhl, P->n, st));
   if (P->kernel == SPX_KERNEL_SEG) {
     // rows of A without nonzeros; the long rows are written by the reduction
     if (P->nempty > 0) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
@@ -1049,29 +1340,27 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
     return SPX_OK;
   }
   if (t0) HIP_TRY(hipEventRecord(t0, st));
+  // The long rows' slices run first, on the same stream: the regular kernel
+  // and the segmented engine each fill the GPU with persistent workgroups,
+  // and sharing it between them measured slower than one after the other.
   const bool longpath = P->nlong > 0 && !dry;
-  if (longpath) {
-    HIP_TRY(hipEventRecord(P->lfork, st));
-    HIP_TRY(hipStreamWaitEvent(P->lstream, P->lfork, 0));
-    HIP_TRY(launch_seg(P, SEG_ALL, x, alpha, beta, y, P->lstream));
-    HIP_TRY(hipEventRecord(P->ldone, P->lstream));
-  }
+  if (longpath) HIP_TRY(launch_seg(P, SEG_ALL, x, alpha, beta, y, st));
   if (P->kernel == SPX_KERNEL_TILE) HIP_TRY(launch_empty_rows(P->d_empty, P->nempty, beta, y, st));
   const bool regular = !(P->nnz_reg == 0 && P->kernel == SPX_KERNEL_TILE);
   if (regular) {
     hipError_t e;
     if (f32)
-      e = run_regular(P, make_args<float, XPair>(P, P->d_val32, P->d_xhl, alpha, beta, y), st);
+      e = run_regular(P, make_args<float>(P, P->d_val32, x, alpha, beta, y), st);
     else
-      e = run_regular(P, make_args<double, double>(P, P->d_val, x, alpha, beta, y), st);
+      e = run_regular(P, make_args<double>(P, P->d_val, x, alpha, beta, y), st);
     HIP_TRY(e);
   }
-  if (longpath) HIP_TRY(hipStreamWaitEvent(st, P->ldone, 0));
   if (t1) HIP_TRY(hipEventRecord(t1, st));
   if (dry) return SPX_OK;
-  HIP_TRY(launch_fixup(P->d_runs, P->nruns, P->d_crow, P->d_cval, alpha, beta, y, st));
-  if (longpath)
-    HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->d_lsp, P->d_lseg, P->d_lrows, alpha, beta, false, y, st));
+  // kernels 1 and 2 write the long rows as empty rows (beta * y), the fold
+  // adds alpha * sum; kernel 3 leaves them to the fold
+  HIP_TRY(launch_finish(P->d_partial, longpath ? P->nlong : 0, P->d_lsp, P->d_lseg, P->d_lrows,
+                        P->kernel == SPX_KERNEL_TILE, P->d_crow, P->d_cval, P->ncarry, alpha, beta, y, st));
   return SPX_OK;
 }
 
@@ -1151,7 +1440,8 @@ int spx_plan_info(spx_plan_t p, int64_t *out, int cap) {
                  p->sigma,  p->lanes,    p->items,
                  p->kernel == SPX_KERNEL_SEG ? p->nsunits : p->nunits, p->ncarry, p->ntiles,
                  p->bytes,  p->precision, p->queues,   p->nlong,  p->long_nnz, p->nblocks,
-                 p->nruns,  p->long_th, p->nstaged_units, p->seg_split ? 1 : 0};
+                 0,  p->long_th, p->nstaged_units, p->seg_split ? 1 : 0,
+                 (int64_t)(p->upload_ms * 1000.0), (int64_t)(p->build_ms * 1000.0)};
   int c = std::min<int>(cap, (int)(sizeof(v) / sizeof(v[0])));
   for (int i = 0; i < c; i++) out[i] = v[i];
   return c;

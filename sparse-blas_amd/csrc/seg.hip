@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
       // latency.  Pairs of 8-byte entries start at the first 16-byte
       // aligned entry; a leading and a trailing single entry are copied
       // through registers.
-      static_assert(sizeof(X) == 8, "x entries are 8 bytes (a double or an FP32 hi/lo pair)");
+      static_assert(sizeof(X) == 8, "x entries are FP64");
       const X *src = a.x + U.cmin;
       lead = (int)(((uintptr_t)src >> 3) & 1);
       const int np = (U.cwin - lead) >> 1;
@@ -254,8 +254,8 @@ static hipError_t seg_launch(const SegArgs<V, X> &a, hipStream_t st) {
 hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st) {
   return stage ? seg_launch<double, double, true>(a, st) : seg_launch<double, double, false>(a, st);
 }
-hipError_t launch_seg_f32(const SegArgs<float, XPair> &a, bool stage, hipStream_t st) {
-  return stage ? seg_launch<float, XPair, true>(a, st) : seg_launch<float, XPair, false>(a, st);
+hipError_t launch_seg_f32(const SegArgs<float, double> &a, bool stage, hipStream_t st) {
+  return stage ? seg_launch<float, double, true>(a, st) : seg_launch<float, double, false>(a, st);
 }
 
 }  // namespace spx

@@ -6,16 +6,80 @@
 namespace spx {
 
 // FP64 product.
-__device__ inline double prod(double v, const double *x, int c) { return v * x[c]; }
+__device__ inline double prod(double v, double xv) { return v * xv; }
 
-// FP32 value times an FP32 (hi, lo) pair, returned as the FP64 sum of the
-// exact product v*hi (split by FMA into ph + eh) and the rounded v*lo.
-__device__ inline double prod(float v, const XPair *x, int c) {
-  XPair p = x[c];
-  float ph = v * p.hi;
-  float eh = __builtin_fmaf(v, p.hi, -ph);
-  float pl = __builtin_fmaf(v, p.lo, eh);
+// Precision form: an FP32 value times x split into FP32 hi = fl(x) and
+// lo = fl(x - hi), returned as the FP64 sum of the exact product v*hi (split
+// by FMA into ph + eh) and the rounded v*lo.  x stays FP64 in memory; the
+// split costs four instructions and no extra pass over x.
+__device__ inline double prod(float v, double xv) {
+  const float hi = (float)xv;
+  const float lo = (float)(xv - (double)hi);
+  const float ph = v * hi;
+  const float eh = __builtin_fmaf(v, hi, -ph);
+  const float pl = __builtin_fmaf(v, lo, eh);
   return (double)ph + (double)pl;
+}
+
+template <typename V>
+__device__ inline double prod(V v, const double *x, int c) {
+  return prod(v, x[c]);
+}
+
+
+// Stage x[cmin, cmin + cwin) in LDS for a workgroup of NT threads: entries go
+// straight from global memory to LDS, 16 bytes per lane
+// (global_load_lds_dwordx4: lane l's 16 bytes land at the wave's LDS address
+// + 16 l), all of the window at once, so the copy costs one memory latency
+// and no registers.  Pairs of entries start at the first 16-byte aligned
+// entry; a leading and a trailing single entry go through registers.
+// Returns `lead`: s_x[lead + k] holds window entry k.  The caller waits
+// (stage_wait) and then synchronises the workgroup before reading s_x, and
+// synchronises before restaging (earlier gathers must be done).
+template <int NT>
+__device__ inline int stage_window(const double *x, int cmin, int cwin, double *s_x, int tid) {
+  const double *src = x + cmin;
+  const int lead = (int)(((uintptr_t)src >> 3) & 1);
+  const int np = (cwin - lead) >> 1;
+  const int wid = tid / kWave, lane = tid % kWave;
+  for (int q0 = wid * kWave; q0 < np; q0 += NT) {
+    const int q = q0 + lane;
+    if (q < np)
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + lead + 2 * q),
+                                       (__attribute__((address_space(3))) void *)(s_x + 2 * lead + 2 * q0), 16, 0,
+                                       0);
+  }
+  if (tid == 0 && lead && cwin > 0) s_x[lead] = src[0];
+  if (tid == kWave % NT && ((cwin - lead) & 1)) s_x[lead + cwin - 1] = src[cwin - 1];
+  return lead;
+}
+// This wave's outstanding loads, the LDS copies included, have completed
+// (vmcnt(0) with expcnt and lgkmcnt left open, gfx9 encoding).
+__device__ inline void stage_wait() { __builtin_amdgcn_s_waitcnt(0x0f70); }
+
+// A unit descriptor of up to 16 int32 fields spread over a wave, one field
+// per lane (lane l holds field l % 16).  Loaded a unit ahead into one vector
+// register, it is read with readlane when the unit starts, so no wait for it
+// delays the unit.
+template <typename T>
+__device__ inline int desc_prefetch(const T *units, int64_t u, int64_t end, int lane) {
+  static_assert(sizeof(T) % 4 == 0 && sizeof(T) <= 64, "descriptor of at most 16 int32 fields");
+  constexpr int F = (int)(sizeof(T) / 4);
+  return u < end ? reinterpret_cast<const int32_t *>(units + u)[(lane % 16) < F ? lane % 16 : 0] : 0;
+}
+__device__ inline int desc_field(int v, int f) { return __builtin_amdgcn_readlane(v, f); }
+
+// Static unit walk of a persistent grid: the workgroups of XCD x (b % kXcd)
+// take x's contiguous share of [0, count) in order, stride = workgroups per
+// XCD, so units that share x entries run together in one L2.
+struct UnitWalk {
+  int64_t u, end, step;
+};
+__device__ inline UnitWalk unit_walk(int64_t count) {
+  const int64_t per = (count + kXcd - 1) / kXcd;
+  const int64_t x = blockIdx.x % kXcd;
+  const int64_t b = x * per, e = b + per < count ? b + per : count;
+  return {b + (int64_t)(blockIdx.x / kXcd), e, (int64_t)(gridDim.x / kXcd)};
 }
 
 // The dispatcher hands workgroup b to XCD b % kXcd.  Units are numbered in

@@ -57,14 +57,58 @@ struct Coord {
   int32_t row, nz;
 };
 
-// A run of carry slots that belong to one split row.
-struct Run {
-  int32_t row, beg, end;
+
+// Kernel 2 (merge path): one unit of the merge of row ends and nonzeros,
+// from coordinate (row r0, nonzero j0) to (r1, j1).  Rows r0 .. r1-1 end in
+// the unit; nonzeros j0 .. j1-1 are consumed by it.
+struct MUnit {
+  int32_t r0, j0, r1, j1;
+  int32_t cmin, cwin;  // x window staged in LDS (cwin = 0: gathers from x)
+  int32_t pad0, pad1;
 };
 
-struct XPair {  // FP32 (hi, lo) split of an FP64 x entry
-  float hi, lo;
+// Kernel 1 (CSR-vector): one unit of consecutive rows [r_lo, r_hi) inside a
+// partition whose nonzeros are [s, e1).  A row cut by the partition's bounds
+// leaves its piece in carry slot 2u (the unit's first row) or 2u + 1 (its
+// last row), u the unit's global index.
+struct VUnit {
+  int32_t r_lo, r_hi, s, e1;
+  int32_t cmin, cwin;  // x window staged in LDS (cwin = 0: gathers from x)
+  int32_t pad0, pad1;
 };
+
+// Kernel 3 (CSR5-style tiles): tiles [lt0, lt0 + nt) of one partition, in
+// chunks of 4 (one per wave).  Carry slots 2gu (the piece of the row open at
+// the unit's start) and 2gu + 1 (the piece open at its end).
+struct TUnit {
+  int32_t gt0;      // global index of the first tile (tile storage, flags)
+  int32_t nt;       // tiles
+  int32_t lt0;      // the first tile's index within its partition
+  int32_t tp0;      // tile pointer entry of the first tile
+  int32_t nz;       // the partition's nonzeros
+  int32_t row_lo;   // the partition's first row
+  int32_t s;        // the partition's first nonzero
+  int32_t gu;       // global unit index
+  int32_t cmin, cwin;
+  int32_t flags;    // kTuSplitHead: the partition's first row began earlier;
+                    // kTuSplitTail: its last row continues after it
+  int32_t last_lr;  // the partition's last local row
+};
+enum { kTuSplitHead = 1, kTuSplitTail = 2 };
+constexpr int kTileUnitChunks = 4;  // chunks of 4 tiles per kernel-3 unit
+constexpr int kTileOut = 512;       // row sums per chunk gathered in LDS before they are stored
+// Kernel 1 with L lanes per row holds kVecRows(L) rows per lane group, so a
+// unit is kBlock / L * kVecRows(L) rows.
+constexpr int kVecRows(int L) { return L <= 8 ? 4 : (L == 16 ? 2 : 1); }
+
+// Carry slots hold pieces of rows split between units, in nonzero order.  A
+// slot's row word is -1 when unused, else the row with two flag bits: the
+// row's first piece (kSlotFirst) and its final piece (kSlotFinal).  Pieces
+// between the two carry neither.  A row's pieces are consecutive among the
+// used slots, so the final piece's fix-up walks back to the first.
+constexpr int32_t kSlotFirst = 1 << 30;
+constexpr uint32_t kSlotFinal = 1u << 31;
+constexpr int32_t kSlotRow = kSlotFirst - 1;
 
 // ---- segmented engine (seg.hip) ----------------------------------------------
 constexpr int kSegStage = 3328;  // most x entries a unit stages in LDS (26 KiB)
@@ -150,7 +194,7 @@ struct SpmmArgs {
   const int32_t *lsp, *lseg, *lrows;
 };
 
-template <typename V, typename X>
+template <typename V>
 struct KArgs {
   const Part *parts;
   const Unit *units;
@@ -159,13 +203,15 @@ struct KArgs {
   const int32_t *rowptr;
   const int32_t *col;
   const V *val;
-  const X *x;
+  const double *x;
   double alpha, beta;
   double *y;
   int32_t *crow;
   double *cval;
-  const Coord *coords;
-  const uint32_t *tile_ptr;
+  const VUnit *vunits;       // kernel 1
+  const MUnit *munits;       // kernel 2
+  const TUnit *tunits;       // kernel 3
+  const uint32_t *tile_ptr;  // kernel 3
   const uint32_t *tile_flags;
   int sigma;
 };
@@ -189,14 +235,9 @@ hipError_t launch_tile_convert(const int32_t *rowptr, const int32_t *col,
                                int32_t *colt, double *valt, float *valt32,
                                hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
-hipError_t launch_split_x(const double *x, XPair *xhl, int64_t n,
-                          hipStream_t st);
 hipError_t launch_to_f32(const double *v, float *o, int64_t n, hipStream_t st);
 hipError_t launch_empty_rows(const int32_t *rows, int64_t nrows, double beta,
                              double *y, hipStream_t st);
-hipError_t launch_fixup(const Run *runs, int64_t nruns, const int32_t *crow,
-                        const double *cval, double alpha, double beta, double *y,
-                        hipStream_t st);
 // With S = sum of partial[lseg[j]], j in [lsp[r], lsp[r+1]): y[rows[r]] =
 // alpha*S + beta*y[rows[r]] (`assign`), else y[rows[r]] += alpha*S.
 hipError_t launch_long_reduce(const double *partial, int nlong, const int32_t *lsp,
@@ -207,12 +248,27 @@ hipError_t launch_find_empty(const int32_t *rowptr, int m, int32_t *flag,
                              hipStream_t st);
 
 // SpMV main kernels.  grid = number of workgroups (persistent when a.queue).
-hipError_t launch_vector_f64(const KArgs<double, double> &a, int lanes, int grid, hipStream_t st);
-hipError_t launch_vector_f32(const KArgs<float, XPair> &a, int lanes, int grid, hipStream_t st);
-hipError_t launch_merge_f64(const KArgs<double, double> &a, int items, int grid, hipStream_t st);
-hipError_t launch_merge_f32(const KArgs<float, XPair> &a, int items, int grid, hipStream_t st);
-hipError_t launch_tile_f64(const KArgs<double, double> &a, int grid, hipStream_t st);
-hipError_t launch_tile_f32(const KArgs<float, XPair> &a, int grid, hipStream_t st);
+hipError_t launch_vector_f64(const KArgs<double> &a, int lanes, int grid, hipStream_t st);
+hipError_t launch_vector_f32(const KArgs<float> &a, int lanes, int grid, hipStream_t st);
+hipError_t launch_merge_f64(const KArgs<double> &a, int items, int grid, hipStream_t st);
+hipError_t launch_merge_f32(const KArgs<float> &a, int items, int grid, hipStream_t st);
+hipError_t launch_tile_f64(const KArgs<double> &a, int grid, hipStream_t st);
+hipError_t launch_tile_f32(const KArgs<float> &a, int grid, hipStream_t st);
+// Fix-up of kernels 1-3: every slot holding a final piece sums its row's
+// pieces (first .. final, in slot order) into y.
+hipError_t launch_slot_fix(const int32_t *crow, const double *cval, int64_t nslots, double alpha, double beta,
+                           double *y, hipStream_t st);
+// The long-row fold (k_long_reduce) and the slot fix-up in one launch.
+hipError_t launch_finish(const double *partial, int nlong, const int32_t *lsp, const int32_t *lseg,
+                         const int32_t *rows, bool assign, const int32_t *crow, const double *cval, int64_t nslots,
+                         double alpha, double beta, double *y, hipStream_t st);
+// Per-unit x windows: for unit u, the int32 fields f[u*stride + off] and
+// f[u*stride + off + 1] hold the unit's nonzero range [lo, hi) of `col` on
+// entry and its staged window (cmin, cwin; cwin = 0 when wider than
+// kSegStage or empty) on return.
+hipError_t launch_unit_window(const int32_t *col, int32_t *f, int stride, int off, int64_t nunits, hipStream_t st);
+// MUnit table of a partition from its coordinates (units + 1 entries).
+hipError_t launch_merge_units(const Coord *coords, int64_t units, MUnit *out, hipStream_t st);
 
 // Multi-RHS SpMM pass on the segmented engine's layout (spmm.hip);
 // spmm_width(kk) = kk rounded up to a power of two (a pass's partial stride).
@@ -221,6 +277,6 @@ hipError_t launch_spmm(const SpmmArgs &a, hipStream_t st);
 
 // Segmented engine: `stage` = some unit stages x in LDS (SEG_STAGE).
 hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st);
-hipError_t launch_seg_f32(const SegArgs<float, XPair> &a, bool stage, hipStream_t st);
+hipError_t launch_seg_f32(const SegArgs<float, double> &a, bool stage, hipStream_t st);
 
 }  // namespace spx

@@ -68,6 +68,7 @@ CAPI = {
     "spx_get_time": (_D, []),
     "spx_get_gpu_availble_mem": (_D, [_I]),
     "spx_set_v1_streams": (_I, [_I]),
+    "spx_dropin_last_times": (_I, [_P, _I]),
     "spx_debug_set_free_mem": (None, [_LL]),
     "spx_malloc": (_P, [_I64]),
     "spx_free": (_I, [_P]),
@@ -229,7 +230,7 @@ class Plan:
         k = lib().spx_plan_info(self.h, out, 32)
         keys = ["m", "n", "nnz", "kernel", "mode", "parts", "sigma", "lanes", "items", "units",
                 "carries", "tiles", "bytes", "precision", "queues", "nlong", "long_nnz", "col_blocks",
-                "runs", "long_row", "staged_units", "split"]
+                "runs", "long_row", "staged_units", "split", "upload_us", "build_us"]
         return {keys[i]: int(out[i]) for i in range(k)}
 
     def export(self, what):
@@ -356,6 +357,16 @@ def spMV_mgpu_v2(m, n, nnz, alpha, val, rowptr, col, x, beta, y, ngpu=1, kernel=
     rc = lib().spx_dropin_v2(m, n, nnz, C.addressof(a), _ptr(val), _ptr(rowptr), _ptr(col), _ptr(x),
                              C.addressof(b), _ptr(y), ngpu, kernel, nb, q)
     return rc, y
+
+
+DROPIN_PHASES = ["plan_ms", "matrix_upload_ms", "xy_upload_ms", "execute_ms", "download_ms", "free_ms", "total_ms"]
+
+
+def dropin_last_times():
+    """Phases (ms of host time) of this thread's last host-pointer drop-in call."""
+    out = (C.c_double * 7)()
+    k = lib().spx_dropin_last_times(out, 7)
+    return {DROPIN_PHASES[i]: float(out[i]) for i in range(k)}
 
 
 def csr5_kernel(m, n, nnz, alpha, val_dev, rowptr32_dev, col_dev, x_dev, beta, y_dev):

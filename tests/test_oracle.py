@@ -279,3 +279,53 @@ def test_harness_alpha_beta_follow_the_values():
 def test_metrics():
     assert oracle.getB(10, 100) == (10 + 1 + 100) * 4 + (200 + 10) * 8
     assert oracle.getFLOP(100) == 200
+
+
+# ---- the precision form's checker: orc_spmv_f32val ------------------------------------
+
+
+def test_f32val_matches_dense_bruteforce_on_float32_values():
+    # the values the checker sees are the FP32 ones, widened exactly to FP64
+    m, n = 37, 29
+    rowptr, col, val, D = rand_csr(m, n, 0.3, rng, empty_frac=0.15)
+    v32 = val.astype(np.float32)
+    D32 = np.zeros_like(D)
+    r = np.repeat(np.arange(m), np.diff(rowptr))
+    D32[r, col] = v32.astype(np.float64)
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, m)
+    for alpha, beta in [(1.0, 0.0), (-0.75, 1.25)]:
+        y = oracle.spmv_f32val(rowptr, col, v32, x, alpha, beta, y0)
+        ref = alpha * (D32 @ x) + beta * y0
+        mag = abs(alpha) * (np.abs(D32) @ np.abs(x)) + abs(beta * y0)
+        assert np.all(np.abs(y - ref) <= 4 * (n + 2) * 2.2e-16 * mag + 1e-300)
+        # and it is not the FP64-valued product: FP32 rounding of the values shows
+        assert np.abs(y - (alpha * (D @ x) + beta * y0)).max() > 1e-9
+
+
+def test_f32val_exact_fraction_sums_keep_x_in_fp64():
+    # FP32-representable values (small dyadic rationals) times x entries that
+    # need all 53 bits: every product and partial sum is exact in FP64, so the
+    # checker must reproduce the rational result exactly.  Rounding x (or the
+    # products) to FP32 anywhere would lose the 2^-40 terms.
+    m, n = 12, 9
+    rr = np.random.default_rng(7)
+    rowptr, col, _, A = rand_csr(m, n, 0.5, rr)
+    val = rr.integers(-16, 17, len(col)) / 8.0
+    val[val == 0] = 0.125
+    x = 1.0 + rr.integers(1, 8, n) * 2.0 ** -40
+    y0 = rr.integers(-4, 5, m) / 4.0
+    y = oracle.spmv_f32val(rowptr, col, val.astype(np.float32), x, 2.0, -0.5, y0)
+    for i in range(m):
+        s = sum(Fraction(float(val[j])) * Fraction(float(x[col[j]])) for j in range(rowptr[i], rowptr[i + 1]))
+        exact = 2 * s - Fraction(1, 2) * Fraction(float(y0[i]))
+        assert Fraction(float(y[i])) == exact, (i, y[i], float(exact))
+
+
+def test_f32val_beta_zero_ignores_nan_y():
+    rowptr = np.array([0, 2, 2, 3], np.int64)
+    col = np.array([0, 2, 1], np.int32)
+    v32 = np.array([0.5, -2.0, 3.0], np.float32)
+    x = np.array([2.0, 1.0, 0.25])
+    y = oracle.spmv_f32val(rowptr, col, v32, x, 1.0, 0.0, np.full(3, np.nan))
+    assert y.tolist() == [0.5, 0.0, 3.0]
