@@ -251,6 +251,20 @@ int spx_dropin_v2(int m, int n, long long nnz, double *alpha, double *csrVal,
 int spx_dropin_csr5_kernel(int m, int n, int nnz, double *alpha,
                            double *csrVal, int *csrRowPtr, int *csrColIndex,
                            double *x, double *beta, double *y);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The arguments and result of spx_dropin_csr5_kernel
+ * (device pointers, int32 CSR, y = alpha*A*x + beta*y), computed by the
+ * merge-path kernel directly on the caller's arrays after a device check of
+ * the row pointer and column indices.  Its unit table and carry slots live
+ * in a file-scope device array of 8 MiB per GPU (address taken once per
+ * device), so m + nnz may be at most 306,774,016; larger inputs fail with -1
+ * and a message in spx_last_error.  Calls that overlap on one GPU share the
+ * array and race: serialise them.  Synchronous; returns 0 or -1. */
+int spx_dropin_csr5_kernel_noalloc(int m, int n, int nnz, double *alpha,
+                                   double *csrVal, int *csrRowPtr,
+                                   int *csrColIndex, double *x, double *beta,
+                                   double *y);
 int spx_get_row_from_index(int n, long long *a, long long idx);
 double spx_get_time(void);
 double spx_get_gpu_availble_mem(int ngpu);

@@ -64,6 +64,7 @@ CAPI = {
     "spx_dropin_v1": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P, _P, _I, _I]),
     "spx_dropin_v2": (_I, [_I, _I, _LL, _P, _P, _P, _P, _P, _P, _P, _I, _I, _LL, _I]),
     "spx_dropin_csr5_kernel": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "spx_dropin_csr5_kernel_noalloc": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "spx_get_row_from_index": (_I, [_I, _P, _LL]),
     "spx_get_time": (_D, []),
     "spx_get_gpu_availble_mem": (_D, [_I]),
