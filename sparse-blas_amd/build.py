@@ -24,7 +24,7 @@ ARCH = os.environ.get("SPX_OFFLOAD_ARCH", "gfx950")
 LIB = os.path.join(HERE, "libspx.so")
 GENLIB = os.path.join(HERE, "libspxgen.so")
 
-DEVICE_SOURCES = ["kernels.hip", "build.hip", "noalloc.hip", "seg.hip", "spmm.hip", "select.hip", "plan.cpp", "dropin.cpp", "select_api.cpp", "mmio.cpp"]
+DEVICE_SOURCES = ["kernels.hip", "build.hip", "noalloc.hip", "seg.hip", "spmm.hip", "select.hip", "plan.cpp", "stage.cpp", "dropin.cpp", "select_api.cpp", "mmio.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Wno-unused-result", "-ffp-contract=fast",

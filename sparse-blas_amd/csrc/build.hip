@@ -87,19 +87,30 @@ hipError_t launch_split_long(const int32_t *col, const double *val, int64_t nnz,
   return hipGetLastError();
 }
 
+// flag[r] = 1 when long row r's columns decrease somewhere: one thread per
+// adjacent pair over all long rows (the row found by a binary search of lrp).
 __global__ void k_long_unsorted(const int32_t *__restrict__ lcol, const int32_t *__restrict__ lrp, int nlong,
                                 int *__restrict__ flag) {
-  // one workgroup per long row: flag[r] = 1 when its columns decrease somewhere
-  const int r = blockIdx.x;
-  if (r >= nlong) return;
-  int bad = 0;
-  for (int64_t k = (int64_t)lrp[r] + 1 + threadIdx.x; k < lrp[r + 1]; k += blockDim.x) bad |= lcol[k - 1] > lcol[k];
-  if (__syncthreads_or(bad) && threadIdx.x == 0) flag[r] = 1;
+  const int64_t total = lrp[nlong];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    if (lcol[k - 1] <= lcol[k]) continue;
+    int lo = 0, hi = nlong;  // the row holding element k: last r with lrp[r] <= k
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (lrp[mid] <= k)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const int r = lo - 1;
+    if (k > lrp[r]) flag[r] = 1;  // not the first element of its row
+  }
 }
 
 hipError_t launch_long_unsorted(const int32_t *lcol, const int32_t *lrp, int nlong, int *flag, hipStream_t st) {
   if (nlong <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_long_unsorted, dim3(nlong), dim3(256), 0, st, lcol, lrp, nlong, flag);
+  hipLaunchKernelGGL(k_long_unsorted, dim3(4096), dim3(256), 0, st, lcol, lrp, nlong, flag);
   return hipGetLastError();
 }
 

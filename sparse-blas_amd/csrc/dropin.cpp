@@ -1,7 +1,9 @@
 // dropin.cpp -- the reference's entry points (SparseBLAS/include/spmv_kernel.h)
 // on one MI355X.  Each call is stateless like the reference's: it builds a
 // device plan from the caller's host arrays, copies x (and y when beta != 0)
-// in, executes, copies y out and frees everything.  Return codes follow the
+// in, executes, copies y out and frees everything.  The copies go through a
+// pinned double buffer (stage.cpp), and the matrix's copy overlaps the
+// host's pass over the row pointer.  Return codes follow the
 // reference: 0 ok, -1 bad argument / memory guard / compute failure, 1 device
 // allocation or copy failure.
 #include <hip/hip_runtime.h>
@@ -16,6 +18,7 @@
 
 #include "../../include/spmv_kernel.h"
 #include "../../include/spx_capi.h"
+#include "spx_internal.h"
 
 namespace {
 
@@ -64,8 +67,8 @@ int run_host(const spx_plan_desc &d, double alpha, const double *x, double beta,
     dy = (double *)spx_malloc((int64_t)d.m * 8);
     if (!dx || !dy) {
       ret = 1;
-    } else if ((d.n > 0 && spx_memcpy_h2d(dx, x, (int64_t)d.n * 8) != SPX_OK) ||
-               (beta != 0.0 && spx_memcpy_h2d(dy, y, (int64_t)d.m * 8) != SPX_OK)) {
+    } else if ((d.n > 0 && spx::stage_h2d(dx, x, (size_t)d.n * 8) != hipSuccess) ||
+               (beta != 0.0 && spx::stage_h2d(dy, y, (size_t)d.m * 8) != hipSuccess)) {
       ret = 1;
     }
     g_times[2] = ms_since(t);
@@ -77,7 +80,7 @@ int run_host(const spx_plan_desc &d, double alpha, const double *x, double beta,
     }
     g_times[3] = ms_since(t);
     t = std::chrono::steady_clock::now();
-    if (ret == 0 && spx_memcpy_d2h(y, dy, (int64_t)d.m * 8) != SPX_OK) ret = 1;
+    if (ret == 0 && spx::stage_d2h(y, dy, (size_t)d.m * 8) != hipSuccess) ret = 1;
     g_times[4] = ms_since(t);
   }
   t = std::chrono::steady_clock::now();

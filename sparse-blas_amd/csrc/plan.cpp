@@ -30,6 +30,8 @@
 #include "spx_build.h"
 #include "spx_internal.h"
 
+#include <thread>
+
 using namespace spx;
 
 namespace {
@@ -492,10 +494,43 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   return SPX_OK;
 }
 
+// Device memory for the duration of a plan build (freed on every path).
+template <typename T>
+struct DevTmp {
+  T *p = nullptr;
+  DevTmp() = default;
+  DevTmp(const DevTmp &) = delete;
+  DevTmp &operator=(const DevTmp &) = delete;
+  ~DevTmp() {
+    if (p) (void)hipFree(p);
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  int alloc(int64_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)(count > 0 ? count : 1));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return SPX_OK;
+  }
+};
+
+// The input's columns and values copied to the device (host input of
+// kernels 1-3), padded by two zero entries.
+struct InputOnDevice {
+  DevTmp<int32_t> col;
+  DevTmp<double> val;
+};
+
 static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
                      const std::vector<int32_t> &long_rows);
 static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
-                     const std::vector<int32_t> &long_rows);
+                     const std::vector<int32_t> &long_rows, InputOnDevice &in);
 
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int m = d->m, n = d->n;
@@ -517,9 +552,44 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   P->queues = d->queues > 0 ? d->queues : 1;
   P->fused_spmm = (d->flags & SPX_FLAG_FUSED_SPMM) != 0;
   if (m == 0) return SPX_OK;
+  const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
+
+  // ---- kernels 1-3 from host arrays: the columns and values go to the
+  // device on a second host thread while this one works on the row pointer
+  InputOnDevice in;
+  std::thread upload;
+  int up_rc = SPX_OK;
+  struct Join {
+    std::thread &t;
+    ~Join() {
+      if (t.joinable()) t.join();
+    }
+  } join{upload};
+  std::string up_msg;
+  if (d->kernel != SPX_KERNEL_SEG && !on_dev && nnz > 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const auto t_up = std::chrono::steady_clock::now();
+    upload = std::thread([&, dev, t_up] {
+      // a bad row pointer found meanwhile takes precedence over whatever
+      // happens here (this thread's error text is handed over on return)
+      int rc = hipSetDevice(dev) == hipSuccess ? SPX_OK : fail(SPX_EDEV, "upload: hipSetDevice failed");
+      if (!rc) rc = in.col.alloc(nnz + 2);
+      if (!rc) rc = in.val.alloc(nnz + 2);
+      hipError_t e = hipSuccess;
+      if (!rc) e = hipMemset(in.col.p + nnz, 0, 2 * sizeof(int32_t));
+      if (!rc && e == hipSuccess) e = hipMemset(in.val.p + nnz, 0, 2 * sizeof(double));
+      if (!rc && e == hipSuccess) e = stage_h2d(in.col.p, d->col, sizeof(int32_t) * (size_t)nnz);
+      if (!rc && e == hipSuccess) e = stage_h2d(in.val.p, d->val, sizeof(double) * (size_t)nnz);
+      if (!rc && e != hipSuccess)
+        rc = fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV, std::string("upload: ") + hipGetErrorString(e));
+      if (rc) up_msg = spx_last_error();
+      up_rc = rc;
+      P->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
+    });
+  }
 
   // ---- host row pointer (validated: monotone, from 0 to nnz) ----
-  const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
   std::vector<int64_t> rp64;
   std::vector<int32_t> rp32;
   const int64_t *src64 = d->rowptr64;
@@ -561,8 +631,10 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   }
   P->nlong = (int)long_rows.size();
 
+  if (upload.joinable()) upload.join();
+  if (up_rc != SPX_OK) return fail(up_rc, up_msg);
   if (d->kernel == SPX_KERNEL_SEG) return build_seg(P, d, rp, long_rows);
-  return build_csr(P, d, rp, long_rows);
+  return build_csr(P, d, rp, long_rows, in);
 }
 
 // Kernel 4: the unit table is built on the host (it needs each unit's column
@@ -690,32 +762,6 @@ static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
     return SPX_OK;
   }
 }
-
-// Device memory for the duration of a plan build (freed on every path).
-template <typename T>
-struct DevTmp {
-  T *p = nullptr;
-  DevTmp() = default;
-  DevTmp(const DevTmp &) = delete;
-  DevTmp &operator=(const DevTmp &) = delete;
-  ~DevTmp() {
-    if (p) (void)hipFree(p);
-  }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-  }
-  int alloc(int64_t count) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)(count > 0 ? count : 1));
-    if (e != hipSuccess) {
-      p = nullptr;
-      return fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    return SPX_OK;
-  }
-};
 
 // The long rows' (column block, row) slices in the segmented engine's
 // layout, built on the device from the long rows' compact arrays (columns
@@ -863,11 +909,10 @@ static int build_long_slices(spx_plan *P, int n, int32_t *lcol, double *lval, co
 // upload and a few passes at HBM speed; the host touches only the row
 // pointer.
 static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
-                     const std::vector<int32_t> &long_rows) {
+                     const std::vector<int32_t> &long_rows, InputOnDevice &in) {
   const int m = d->m, n = d->n;
   const int64_t nnz = d->nnz;
   const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
-  const hipMemcpyKind in_kind = on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   // FP32 values in CSR order (kernel 3 converts while it builds its tiles)
   const bool f32_csr = P->precision == SPX_PREC_F32 && d->kernel != SPX_KERNEL_TILE;
   int rc;
@@ -900,64 +945,62 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
   }
 
   // ---- the regular CSR on the device, padded by two zero entries ----
+  // the input's columns and values in device memory: the caller's (device
+  // input) or the copies made while the row pointer was read (host input)
+  const int32_t *icol = on_dev ? d->col : in.col.p;
+  const double *ival = on_dev ? d->val : in.val.p;
   DevTmp<int> d_bad;
   if ((rc = d_bad.alloc(1))) return rc;
   HIP_TRY(hipMemset(d_bad.p, 0, sizeof(int)));
-  if ((rc = P->alloc(&P->d_col, nnzr + 2))) return rc;
-  HIP_TRY(hipMemset(P->d_col + nnzr, 0, 2 * sizeof(int32_t)));
-  if (f32_csr) {
-    if ((rc = P->alloc(&P->d_val32, nnzr + 2))) return rc;
-    HIP_TRY(hipMemset(P->d_val32 + nnzr, 0, 2 * sizeof(float)));
+  if (nnz > 0) HIP_TRY(launch_check_cols(icol, nnz, n, d_bad.p, 0));
+  DevTmp<int32_t> lcol;
+  DevTmp<double> lval;
+  if (P->nlong == 0 && in.col.p) {
+    // the copies become the plan's arrays
+    P->d_col = in.col.p;
+    in.col.p = nullptr;
+    P->bytes += (int64_t)sizeof(int32_t) * (nnzr + 2);
+    if (!f32_csr) {
+      P->d_val = in.val.p;
+      in.val.p = nullptr;
+      P->bytes += (int64_t)sizeof(double) * (nnzr + 2);
+    }
   } else {
-    if ((rc = P->alloc(&P->d_val, nnzr + 2))) return rc;
-    HIP_TRY(hipMemset(P->d_val + nnzr, 0, 2 * sizeof(double)));
+    if ((rc = P->alloc(&P->d_col, nnzr + 2))) return rc;
+    HIP_TRY(hipMemset(P->d_col + nnzr, 0, 2 * sizeof(int32_t)));
   }
-  DevTmp<int32_t> craw, lcol;
-  DevTmp<double> vraw, lval;
-  const auto t_up = std::chrono::steady_clock::now();
+  if (!P->d_val) {
+    if (f32_csr) {
+      if ((rc = P->alloc(&P->d_val32, nnzr + 2))) return rc;
+      HIP_TRY(hipMemset(P->d_val32 + nnzr, 0, 2 * sizeof(float)));
+    } else {
+      if ((rc = P->alloc(&P->d_val, nnzr + 2))) return rc;
+      HIP_TRY(hipMemset(P->d_val + nnzr, 0, 2 * sizeof(double)));
+    }
+  }
   if (P->nlong == 0) {
     if (nnz > 0) {
-      HIP_TRY(hipMemcpy(P->d_col, d->col, sizeof(int32_t) * nnz, in_kind));
-      if (f32_csr) {
-        const double *v = d->val;
-        if (!on_dev) {
-          if ((rc = vraw.alloc(nnz))) return rc;
-          HIP_TRY(hipMemcpy(vraw.p, d->val, sizeof(double) * nnz, hipMemcpyHostToDevice));
-          v = vraw.p;
-        }
-        HIP_TRY(launch_to_f32(v, P->d_val32, nnz, 0));
-      } else {
-        HIP_TRY(hipMemcpy(P->d_val, d->val, sizeof(double) * nnz, in_kind));
-      }
+      if (icol != P->d_col) HIP_TRY(hipMemcpy(P->d_col, icol, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice));
+      if (f32_csr)
+        HIP_TRY(launch_to_f32(ival, P->d_val32, nnz, 0));
+      else if (ival != P->d_val)
+        HIP_TRY(hipMemcpy(P->d_val, ival, sizeof(double) * nnz, hipMemcpyDeviceToDevice));
     }
-    P->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
-    HIP_TRY(launch_check_cols(P->d_col, nnz, n, d_bad.p, 0));
   } else {
-    const int32_t *col = d->col;
-    const double *val = d->val;
-    if (!on_dev) {
-      if ((rc = craw.alloc(nnz)) || (rc = vraw.alloc(nnz))) return rc;
-      HIP_TRY(hipMemcpy(craw.p, d->col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(vraw.p, d->val, sizeof(double) * nnz, hipMemcpyHostToDevice));
-      col = craw.p;
-      val = vraw.p;
-    }
-    P->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
-    HIP_TRY(launch_check_cols(col, nnz, n, d_bad.p, 0));
     DevTmp<LongRange> d_lr;
     if ((rc = d_lr.alloc(P->nlong)) || (rc = lcol.alloc(ln)) || (rc = lval.alloc(ln))) return rc;
     HIP_TRY(hipMemcpy(d_lr.p, lr.data(), sizeof(LongRange) * lr.size(), hipMemcpyHostToDevice));
-    HIP_TRY(launch_split_long(col, val, nnz, d_lr.p, P->nlong, P->d_col, f32_csr ? nullptr : P->d_val,
+    HIP_TRY(launch_split_long(icol, ival, nnz, d_lr.p, P->nlong, P->d_col, f32_csr ? nullptr : P->d_val,
                               f32_csr ? P->d_val32 : nullptr, lcol.p, lval.p, 0));
-    HIP_TRY(hipDeviceSynchronize());
   }
+  HIP_TRY(hipDeviceSynchronize());
   {
     int bad = 0;
     HIP_TRY(hipMemcpy(&bad, d_bad.p, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) return fail(SPX_EARG, "column index out of range");
   }
-  craw.reset();
-  vraw.reset();
+  in.col.reset();
+  in.val.reset();
 
   // ---- long rows: (column block, row) slices for the segmented engine ----
   if (P->nlong > 0) {
@@ -1594,8 +1637,14 @@ extern "C" int spx_batch_create(spx_plan_t *out, int count, const spx_batch_item
   int64_t r0 = 0, c0 = 0, z0 = 0;
   for (int i = 0; i < count; i++) {
     const spx_batch_item &it = items[i];
+    // each member's row pointer runs monotonically from 0 to its nnz (a member
+    // with no rows owns no nonzeros), so no member's nonzeros land in another
+    if (it.m == 0 && it.nnz != 0) return fail(SPX_EARG, "batch item without rows has nonzeros");
     if (it.m > 0 && (it.rowptr[0] != 0 || it.rowptr[it.m] != it.nnz)) return fail(SPX_EARG, "bad batch row pointer");
-    for (int r = 1; r <= it.m; r++) rp[r0 + r] = z0 + it.rowptr[r];
+    for (int r = 1; r <= it.m; r++) {
+      if (it.rowptr[r] < it.rowptr[r - 1]) return fail(SPX_EARG, "batch row pointer not monotone");
+      rp[r0 + r] = z0 + it.rowptr[r];
+    }
     for (int64_t k = 0; k < it.nnz; k++) {
       if (it.col[k] < 0 || it.col[k] >= it.n) return fail(SPX_EARG, "batch column index out of range");
       col[z0 + k] = (int32_t)(it.col[k] + c0);

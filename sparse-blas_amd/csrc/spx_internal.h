@@ -216,6 +216,11 @@ struct KArgs {
   int sigma;
 };
 
+// Copies between pageable host memory and the device through a pinned double
+// buffer (stage.cpp); small copies go straight through hipMemcpy.
+hipError_t stage_h2d(void *dst, const void *src, size_t bytes);
+hipError_t stage_d2h(void *dst, const void *src, size_t bytes);
+
 // The thread's message for spx_last_error (plan.cpp).
 void set_last_error(const std::string &msg);
 
