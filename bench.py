@@ -280,8 +280,10 @@ def main():
                     help="executes timed on the uniform-column variant of the matrix (0 = skip)")
     ap.add_argument("--dropin-reps", type=int, default=3,
                     help="calls of the host-pointer drop-in timed end to end (0 = skip)")
-    ap.add_argument("--cold-reps", type=int, default=20,
+    ap.add_argument("--cold-reps", type=int, default=30,
                     help="executes timed with L2/MALL flushed before each (0 = skip)")
+    ap.add_argument("--versions-reps", type=int, default=20,
+                    help="executes timed per reference kernel version 1-3 on the same matrix (0 = skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -335,17 +337,39 @@ def main():
     # around each execute and around its SpMV kernel; the library reads the
     # host clock from the first timed enqueue to the end of the last execute
     spx.synchronize()
-    tot_ms, ker_ms, wall_ms = plan.time(xd, yd, alpha, beta, warmup=0, reps=a.steps, wall=True)
+    ex_arr, ker_arr, wall_ms = plan.time_each(xd, yd, alpha, beta, warmup=0, reps=a.steps)
     spx.synchronize()
+    tot_ms, ker_ms = float(ex_arr.mean()), float(ker_arr.mean())
+    ker_med = float(np.median(ker_arr))
     log(f"timed {a.steps} steps: wall {wall_ms:.4f} ms/step, events {tot_ms:.4f} ms/execute, "
-        f"kernel {ker_ms:.4f} ms")
+        f"kernel mean {ker_ms:.4f} / median {ker_med:.4f} ms")
 
-    # the same kernel with L2 and MALL cold: a 512 MiB buffer is overwritten
-    # before every timed execute (outside its events)
-    cold_ms = None
+    # the same kernel with L2 and MALL cold (the HBM-labelled figure): before
+    # every execute a 1 GiB buffer is read, which evicts both caches and leaves
+    # no dirty lines; for comparison, a 512 MiB buffer overwritten instead
+    # (its dirty lines are written back while the kernel runs)
+    cold = None
     if a.cold_reps > 0:
-        _, cold_ms = plan.time(xd, yd, alpha, beta, warmup=1, reps=a.cold_reps, flush_bytes=512 << 20)
-        log(f"cold caches: kernel {cold_ms:.4f} ms ({a.cold_reps} executes, 512 MiB overwritten before each)")
+        _, kc, _ = plan.time_each(xd, yd, alpha, beta, warmup=1, reps=a.cold_reps, flush_bytes=1 << 30)
+        _, kw, _ = plan.time_each(xd, yd, alpha, beta, warmup=1, reps=a.cold_reps, flush_bytes=-(512 << 20))
+        cold = {"mean": float(kc.mean()), "median": float(np.median(kc)), "write_flush_median": float(np.median(kw))}
+        log(f"cold caches ({a.cold_reps} executes each): kernel mean {cold['mean']:.4f} / median "
+            f"{cold['median']:.4f} ms after a 1 GiB read; median {cold['write_flush_median']:.4f} ms after a "
+            f"512 MiB overwrite")
+
+    # the reference's kernel versions on the same matrix (the drop-in's
+    # kernels), each timed warm: their SpMV kernels and the whole execute
+    versions = None
+    if a.versions_reps > 0 and a.kernel == 4:
+        versions = {}
+        for kv, kw_ in ((1, {}), (2, {"items": 8}), (3, {})):
+            Pv = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=kv, **kw_)
+            ev, kk, wv = Pv.time_each(xd, yd, alpha, beta, warmup=3, reps=a.versions_reps)
+            versions[f"kernel{kv}"] = {"ms_per_step": round(wv, 5), "execute_ms_median": round(float(np.median(ev)), 5),
+                                       "kernels_ms_median": round(float(np.median(kk)), 5),
+                                       "GBs": round(W.bytes_model(m, n, nnz, 8, beta != 0.0) / (wv * 1e-3) / 1e9, 1)}
+            log(f"kernel version {kv}: wall {wv:.4f} ms/step, execute median {np.median(ev):.4f} ms")
+            Pv.close()
 
     # the same sizes with every row's columns uniform at random (no locality
     # at all: the gathers, not the matrix stream, bound it)
@@ -395,7 +419,11 @@ def main():
     B = W.bytes_model(m, n, nnz, 8, beta != 0.0)
     y = yd.download(np.float64, m)
     value = B / (wall_ms * 1e-3) / 1e9
-    achieved = B / (ker_ms * 1e-3) / 1e9
+    # the roofline figure: the SpMV kernel's mean duration with cold caches
+    # when measured, else the warm mean
+    k_roof = cold["mean"] if cold else ker_ms
+    achieved = B / (k_roof * 1e-3) / 1e9
+    traffic = profiled_traffic(a.kernel, info["items"], a.window)
     out = {
         "metric": "SpMV GB/s & % HBM peak; learned+FP32-storage vs plain FP64 at stated rel. err",
         "value": round(value, 2),
@@ -425,10 +453,19 @@ def main():
         "ref_getB_gbps": round(W.ref_getB(m, nnz) / (wall_ms * 1e-3) / 1e9, 2),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(a.kernel, info["items"], a.window),
-            "bytes_per_launch": B, "kernel_ms": round(ker_ms, 5),
-            "kernel_ms_cold_caches": None if cold_ms is None else round(cold_ms, 5),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_launch": B,
+            "basis": ("algorithmic bytes / mean SpMV-kernel time with L2 and MALL cold (a 1 GiB read before each "
+                      "execute); the long-row fold after the kernel is outside it") if cold else
+                     "algorithmic bytes / mean SpMV-kernel time, warm",
+            "kernel_ms_cold_mean": None if cold is None else round(cold["mean"], 5),
+            "kernel_ms_cold_median": None if cold is None else round(cold["median"], 5),
+            "kernel_ms_cold_write_flush_median": None if cold is None else round(cold["write_flush_median"], 5),
+            "kernel_ms_warm_mean": round(ker_ms, 5), "kernel_ms_warm_median": round(ker_med, 5),
+            "frac_warm_median": round(B / (ker_med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic_frac_cold": None if (traffic is None) else round(traffic / (k_roof * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
+        "kernel_versions": versions,
     }
 
     if not a.no_cpu:

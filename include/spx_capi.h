@@ -140,15 +140,23 @@ int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap);
 /* Average time (ms) of each of `reps` executions after `warmup` untimed
  * ones, enqueued back-to-back on one stream with HIP events around each.
  * out_ms[0] = whole execute (events around each execute), out_ms[1] = the
- * dominant SpMV kernel alone (partition kernels, fixup and long-row
- * reduction excluded), out_ms[2] = host wall clock from the first timed
- * enqueue to the end of the last execute, per execute.  out_ms holds 3.
- * If flush_bytes > 0 a buffer of that size is overwritten before every
- * timed execute so L2/MALL start cold (outside the events, inside the wall
- * clock). */
+ * SpMV kernels alone (the partition setup, fix-up and long-row fold
+ * excluded), out_ms[2] = host wall clock from the first timed enqueue to the
+ * end of the last execute, per execute.  out_ms holds 3.
+ * flush_bytes > 0: before every timed execute a buffer of that size is read,
+ * which evicts L2 and MALL and leaves no dirty lines behind; flush_bytes < 0:
+ * a buffer of -flush_bytes is overwritten instead (its dirty lines are then
+ * written back while the next execute runs).  The flush lies outside the
+ * events and inside the wall clock. */
 int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta,
                   double *y, int warmup, int reps, int64_t flush_bytes,
                   double *out_ms);
+/* The same timing with every execute's figures kept: exec_ms[i] and
+ * kernel_ms[i] (arrays of `reps`) as out_ms[0] and [1] above, *wall_ms (may
+ * be NULL) the host wall clock per execute. */
+int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta,
+                       double *y, int warmup, int reps, int64_t flush_bytes,
+                       double *exec_ms, double *kernel_ms, double *wall_ms);
 
 /* ---- batched form -----------------------------------------------------------
  * Many independent SpMVs y_i = alpha*A_i*x_i + beta*y_i as ONE plan: the

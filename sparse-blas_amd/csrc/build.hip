@@ -234,4 +234,22 @@ hipError_t launch_scale_y(double beta, double *y, int64_t m, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Cache flush for cold-cache timing: read `n` 16-byte words (evicting L2
+// and MALL without leaving dirty lines behind) and keep one value.
+__global__ void k_flush_read(const uint4 *__restrict__ p, int64_t n, unsigned *__restrict__ sink) {
+  unsigned acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 v = p[i];  // a cached load: it must displace what L2 and MALL hold
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (acc == 0x9e3779b9u) *sink = acc;  // keeps the loads; practically never stores
+}
+
+hipError_t launch_flush_read(const void *buf, int64_t bytes, unsigned *sink, hipStream_t st) {
+  const int64_t n = bytes / 16;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_flush_read, dim3(4096), dim3(256), 0, st, static_cast<const uint4 *>(buf), n, sink);
+  return hipGetLastError();
+}
+
 }  // namespace spx

@@ -1527,9 +1527,9 @@ int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap) {
   return bytes;
 }
 
-int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, double *y, int warmup, int reps,
-                  int64_t flush_bytes, double *out_ms) {
-  if (!p || reps <= 0 || !out_ms) return fail(SPX_EARG, "bad timing arguments");
+int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta, double *y, int warmup, int reps,
+                       int64_t flush_bytes, double *exec_ms, double *kernel_ms, double *wall_ms) {
+  if (!p || reps <= 0 || !exec_ms || !kernel_ms) return fail(SPX_EARG, "bad timing arguments");
   // Every execute gets its own event pairs, so the timed executes run
   // back-to-back on one stream with a single host wait at the end.
   hipStream_t st = nullptr;
@@ -1539,37 +1539,53 @@ int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, doub
   for (auto &e : ev)
     if (rc == SPX_OK && hipEventCreate(&e) != hipSuccess) rc = fail(SPX_EDEV, "hipEventCreate failed");
   void *flush = nullptr;
-  if (rc == SPX_OK && flush_bytes > 0 && hipMalloc(&flush, (size_t)flush_bytes) != hipSuccess)
-    rc = fail(SPX_EMEM, "hipMalloc of the cache-flush buffer failed");
+  const int64_t fb = flush_bytes < 0 ? -flush_bytes : flush_bytes;
+  if (rc == SPX_OK && fb > 0) {
+    if (hipMalloc(&flush, (size_t)fb + 256) != hipSuccess)
+      rc = fail(SPX_EMEM, "hipMalloc of the cache-flush buffer failed");
+    else if (hipMemset(flush, 0, (size_t)fb + 256) != hipSuccess)
+      rc = fail(SPX_EDEV, "hipMemset of the cache-flush buffer failed");
+  }
+  unsigned *sink = flush ? reinterpret_cast<unsigned *>(static_cast<char *>(flush) + fb) : nullptr;
   for (int i = 0; i < warmup && rc == SPX_OK; i++) rc = execute_impl(p, alpha, x, beta, y, st, nullptr, nullptr, false);
   if (rc == SPX_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(SPX_EDEV, "stream synchronize failed");
   const auto w0 = std::chrono::steady_clock::now();
   for (int i = 0; i < reps && rc == SPX_OK; i++) {
-    if (flush) (void)hipMemsetAsync(flush, i & 0xff, (size_t)flush_bytes, st);
+    if (flush && flush_bytes > 0) (void)launch_flush_read(flush, fb, sink, st);
+    if (flush && flush_bytes < 0) (void)hipMemsetAsync(flush, i & 0xff, (size_t)fb, st);
     (void)hipEventRecord(ev[4 * i], st);
     rc = execute_impl(p, alpha, x, beta, y, st, ev[4 * i + 1], ev[4 * i + 2], false);
     (void)hipEventRecord(ev[4 * i + 3], st);
   }
   if (hipStreamSynchronize(st) != hipSuccess && rc == SPX_OK) rc = fail(SPX_EDEV, "stream synchronize failed");
   const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
-  double tot = 0, ker = 0;
   if (rc == SPX_OK) {
     for (int i = 0; i < reps; i++) {
       float t = 0, tk = 0;
       (void)hipEventElapsedTime(&t, ev[4 * i], ev[4 * i + 3]);
       (void)hipEventElapsedTime(&tk, ev[4 * i + 1], ev[4 * i + 2]);
-      tot += t;
-      ker += tk;
+      exec_ms[i] = t;
+      kernel_ms[i] = tk;
     }
+    if (wall_ms) *wall_ms = wall / reps;
   }
   if (flush) (void)hipFree(flush);
   for (auto e : ev)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(st);
+  return rc;
+}
+
+int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, double *y, int warmup, int reps,
+                  int64_t flush_bytes, double *out_ms) {
+  if (!p || reps <= 0 || !out_ms) return fail(SPX_EARG, "bad timing arguments");
+  std::vector<double> e((size_t)reps), k((size_t)reps);
+  double wall = 0.0;
+  const int rc = spx_plan_time_each(p, alpha, x, beta, y, warmup, reps, flush_bytes, e.data(), k.data(), &wall);
   if (rc != SPX_OK) return rc;
-  out_ms[0] = tot / reps;
-  out_ms[1] = ker / reps;
-  out_ms[2] = wall / reps;
+  out_ms[0] = std::accumulate(e.begin(), e.end(), 0.0) / reps;
+  out_ms[1] = std::accumulate(k.begin(), k.end(), 0.0) / reps;
+  out_ms[2] = wall;
   return SPX_OK;
 }
 

@@ -31,4 +31,7 @@ hipError_t launch_seg_fill(const SegUnit *units, int64_t nunits, const int32_t *
 // y = beta * y over m rows (beta == 0 writes zeros without reading y).
 hipError_t launch_scale_y(double beta, double *y, int64_t m, hipStream_t st);
 
+// Read `bytes` of buf (a cache flush that leaves no dirty lines).
+hipError_t launch_flush_read(const void *buf, int64_t bytes, unsigned *sink, hipStream_t st);
+
 }  // namespace spx

@@ -52,6 +52,7 @@ CAPI = {
     "spx_plan_info": (_I, [_P, _P, _I]),
     "spx_plan_export": (_I64, [_P, _I, _P, _I64]),
     "spx_plan_time": (_I, [_P, _D, _P, _D, _P, _I, _I, _I64, _P]),
+    "spx_plan_time_each": (_I, [_P, _D, _P, _D, _P, _I, _I, _I64, _P, _P, _P]),
     "spx_mm_read": (_I, [C.c_char_p, _P, _P, _P, _P, _P, _P]),
     "spx_mm_free": (None, [_P]),
     "spx_sel_param_count": (_I, []),
@@ -253,6 +254,17 @@ class Plan:
         out = (C.c_double * 3)()
         check(lib().spx_plan_time(self.h, alpha, x_dev.ptr, beta, y_dev.ptr, warmup, reps, flush_bytes, out))
         return (out[0], out[1], out[2]) if wall else (out[0], out[1])
+
+    def time_each(self, x_dev, y_dev, alpha=1.0, beta=0.0, warmup=5, reps=20, flush_bytes=0):
+        """(execute ms array, SpMV kernel ms array, wall ms per execute) of `reps`
+        back-to-back executes; flush_bytes > 0 reads a buffer of that size
+        before each (cold L2 and MALL), < 0 overwrites one."""
+        e = np.zeros(reps)
+        k = np.zeros(reps)
+        w = C.c_double()
+        check(lib().spx_plan_time_each(self.h, alpha, x_dev.ptr, beta, y_dev.ptr, warmup, reps, flush_bytes,
+                                       e.ctypes.data, k.ctypes.data, C.byref(w)))
+        return e, k, w.value
 
     def close(self):
         if getattr(self, "h", None):
