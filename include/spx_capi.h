@@ -292,6 +292,12 @@ int spx_set_v1_streams(int parts);
  * free device memory instead of hipMemGetInfo (-1 restores the query). */
 void spx_debug_set_free_mem(long long bytes);
 
+/* Host-side seam for tests: when after >= 0, that many more device
+ * allocations of the library succeed and the next one fails as out of
+ * memory (the drop-ins then return 1, as the reference does on a failed
+ * cudaMalloc, SparseBLAS/src/dspmv_mgpu_baseline.cu:119-153); -1 disarms. */
+void spx_debug_fail_alloc(long long after);
+
 /* ---- device memory helpers (so FFI callers need no HIP bindings) -------- */
 void *spx_malloc(int64_t bytes);
 int spx_free(void *p);

@@ -61,6 +61,43 @@ def build_lib():
     return LIB
 
 
+ASAN_DIR = os.path.join(HERE, "_asan")
+ASAN_LIB = os.path.join(ASAN_DIR, "libspx.so")
+# Host code instrumented with AddressSanitizer and UndefinedBehaviorSanitizer
+# (device code is built as usual): the CPU tests run against it with the
+# clang ASan runtime preloaded (tests/test_sanitized_host.py).
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+
+
+def asan_runtime():
+    """The clang ASan runtime hipcc links the sanitized host code against."""
+    base = "/opt/rocm/lib/llvm/lib/clang"
+    for v in sorted(os.listdir(base)) if os.path.isdir(base) else []:
+        p = os.path.join(base, v, "lib", "linux", "libclang_rt.asan-x86_64.so")
+        if os.path.exists(p):
+            return p
+    return None
+
+
+def build_asan():
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    objs = []
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + ["-O1", "-g"] + SAN_HOST
+    for src in DEVICE_SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(ASAN_DIR, os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        _run([HIPCC, *flags, *lang, "-c", path, "-o", obj])
+        objs.append(obj)
+    tmp = ASAN_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libsan", *SAN_HOST, "-o", tmp, *objs])
+    os.replace(tmp, ASAN_LIB)
+    for o in objs:
+        os.remove(o)
+    return ASAN_LIB
+
+
 def build_gen():
     tmp = GENLIB + ".tmp"
     _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall", "-o", tmp,
@@ -99,7 +136,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true")
     ap.add_argument("--gen", action="store_true")
+    ap.add_argument("--asan", action="store_true", help="only the sanitized host build")
     a = ap.parse_args(argv)
+    if a.asan:
+        print("built", build_asan())
+        return
     both = not (a.lib or a.gen)
     if a.gen or both:
         print("built", build_gen())
@@ -108,6 +149,8 @@ def main(argv=None):
     if both or (os.path.exists(LIB) and os.path.exists(GENLIB)):
         print("built", build_harness())
         print("built", build_handle_check())
+    if both:
+        print("built", build_asan())
 
 
 if __name__ == "__main__":

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <functional>
 #include <cmath>
@@ -37,6 +38,10 @@ using namespace spx;
 namespace {
 
 thread_local std::string g_last_error;
+
+// Test seam: when >= 0, that many more device allocations succeed and the
+// next one reports hipErrorOutOfMemory (then the seam disarms).
+std::atomic<long long> g_fail_alloc{-1};
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -162,7 +167,7 @@ struct spx_plan {
   template <typename T>
   int alloc(T **p, int64_t count) {
     if (count <= 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
+    hipError_t e = dev_malloc((void **)p, sizeof(T) * (size_t)count);
     if (e != hipSuccess) {
       *p = nullptr;
       return fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV,
@@ -200,8 +205,25 @@ const char *spx_last_error(void) { return g_last_error.c_str(); }
 
 void spx::set_last_error(const std::string &msg) { g_last_error = msg; }
 
+// Every device allocation of the library goes through here.
+hipError_t spx::dev_malloc(void **p, size_t bytes) {
+  long long left = g_fail_alloc.load();
+  while (left >= 0) {
+    if (g_fail_alloc.compare_exchange_weak(left, left - 1)) {
+      if (left == 0) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+      }
+      break;
+    }
+  }
+  return hipMalloc(p, bytes);
+}
+
 extern "C" {
 const char *spx_version(void) { return "spx 0.2 gfx950"; }
+
+void spx_debug_fail_alloc(long long after) { g_fail_alloc = after < 0 ? -1 : after; }
 
 }  // extern "C"
 
@@ -511,7 +533,7 @@ struct DevTmp {
   int alloc(int64_t count) {
     if (p) (void)hipFree(p);
     p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)(count > 0 ? count : 1));
+    hipError_t e = dev_malloc((void **)&p, sizeof(T) * (size_t)(count > 0 ? count : 1));
     if (e != hipSuccess) {
       p = nullptr;
       return fail(e == hipErrorOutOfMemory ? SPX_EMEM : SPX_EDEV, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -556,16 +578,18 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
 
   // ---- kernels 1-3 from host arrays: the columns and values go to the
   // device on a second host thread while this one works on the row pointer
+  // (everything the upload thread touches is declared before the guard that
+  // joins it, so an early return joins the thread before any of it dies)
   InputOnDevice in;
   std::thread upload;
   int up_rc = SPX_OK;
+  std::string up_msg;
   struct Join {
     std::thread &t;
     ~Join() {
       if (t.joinable()) t.join();
     }
   } join{upload};
-  std::string up_msg;
   if (d->kernel != SPX_KERNEL_SEG && !on_dev && nnz > 0) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
@@ -573,7 +597,9 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     upload = std::thread([&, dev, t_up] {
       // a bad row pointer found meanwhile takes precedence over whatever
       // happens here (this thread's error text is handed over on return)
-      int rc = hipSetDevice(dev) == hipSuccess ? SPX_OK : fail(SPX_EDEV, "upload: hipSetDevice failed");
+      int cur = -1, rc = SPX_OK;
+      if (hipGetDevice(&cur) == hipSuccess && cur != dev && hipSetDevice(dev) != hipSuccess)
+        rc = fail(SPX_EDEV, "upload: hipSetDevice failed");
       if (!rc) rc = in.col.alloc(nnz + 2);
       if (!rc) rc = in.val.alloc(nnz + 2);
       hipError_t e = hipSuccess;
@@ -1541,7 +1567,7 @@ int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta,
   void *flush = nullptr;
   const int64_t fb = flush_bytes < 0 ? -flush_bytes : flush_bytes;
   if (rc == SPX_OK && fb > 0) {
-    if (hipMalloc(&flush, (size_t)fb + 256) != hipSuccess)
+    if (dev_malloc(&flush, (size_t)fb + 256) != hipSuccess)
       rc = fail(SPX_EMEM, "hipMalloc of the cache-flush buffer failed");
     else if (hipMemset(flush, 0, (size_t)fb + 256) != hipSuccess)
       rc = fail(SPX_EDEV, "hipMemset of the cache-flush buffer failed");
@@ -1593,7 +1619,7 @@ int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta, doub
 
 void *spx_malloc(int64_t bytes) {
   void *p = nullptr;
-  if (hipMalloc(&p, (size_t)(bytes > 0 ? bytes : 1)) != hipSuccess) {
+  if (dev_malloc(&p, (size_t)(bytes > 0 ? bytes : 1)) != hipSuccess) {
     g_last_error = "hipMalloc failed";
     return nullptr;
   }

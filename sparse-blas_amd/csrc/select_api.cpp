@@ -44,7 +44,7 @@ int spx_sel_features(int m, int n, int64_t nnz, const int32_t *rowptr, const int
     return SPX_EARG;
   }
   unsigned long long *part = nullptr;
-  hipError_t e = hipMalloc((void **)&part, sizeof(unsigned long long) * 32 * kFeatBlocks);
+  hipError_t e = spx::dev_malloc((void **)&part, sizeof(unsigned long long) * 32 * kFeatBlocks);
   if (e != hipSuccess) return set_err(e);
   const int64_t th = long_row > 0 ? long_row : ((int64_t)1 << 62);
   e = launch_features(rowptr, col, m, n, nnz, th, part, kFeatBlocks, feat, (hipStream_t)stream);

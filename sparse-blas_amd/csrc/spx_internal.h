@@ -221,6 +221,9 @@ struct KArgs {
 hipError_t stage_h2d(void *dst, const void *src, size_t bytes);
 hipError_t stage_d2h(void *dst, const void *src, size_t bytes);
 
+// hipMalloc behind the allocation-failure test seam (spx_debug_fail_alloc).
+hipError_t dev_malloc(void **p, size_t bytes);
+
 // The thread's message for spx_last_error (plan.cpp).
 void set_last_error(const std::string &msg);
 

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "libspx.so")
+# SPX_LIBRARY selects another build of libspx.so (the sanitized host build
+# in sparse-blas_amd/_asan/ for the CPU tests); the default is the in-tree one
+LIB_PATH = os.environ.get("SPX_LIBRARY") or os.path.join(_PKG, "libspx.so")
 GEN_PATH = os.path.join(_PKG, "libspxgen.so")
 
 OK, EARG, EMEM, EDEV, EGUARD, ERANGE = 0, -1, -2, -3, -4, -5
@@ -72,6 +74,7 @@ CAPI = {
     "spx_set_v1_streams": (_I, [_I]),
     "spx_dropin_last_times": (_I, [_P, _I]),
     "spx_debug_set_free_mem": (None, [_LL]),
+    "spx_debug_fail_alloc": (None, [_LL]),
     "spx_malloc": (_P, [_I64]),
     "spx_free": (_I, [_P]),
     "spx_memcpy_h2d": (_I, [_P, _P, _I64]),

@@ -107,6 +107,32 @@ def test_memory_guard_rejects_through_the_seam():
         L.spx_debug_set_free_mem(-1)
 
 
+def test_dropin_returns_1_when_a_device_allocation_fails():
+    # the reference returns 1 when cudaMalloc or a copy fails
+    # (SB/src/dspmv_mgpu_baseline.cu:119-153); the seam fails the next
+    # allocation before any device is touched, so this runs without a GPU
+    m, n, nnz, rp, c, v, x, y = tiny()
+    L = spx.lib()
+    try:
+        L.spx_debug_set_free_mem(1 << 40)  # the memory guard passes
+        cases = [lambda: spx.spMV_mgpu_baseline(m, n, nnz, 1.0, v, rp, c, x, 0.0, y)]
+        cases += [lambda k=k: spx.spMV_mgpu_v1(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=k) for k in (1, 2, 3)]
+        cases += [lambda k=k: spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=k, nb=2, q=2)
+                  for k in (1, 2, 3)]
+        for call in cases:
+            L.spx_debug_fail_alloc(0)
+            assert call()[0] == 1
+            assert b"out of memory" in L.spx_last_error().lower() or b"memory" in L.spx_last_error().lower()
+        # a plan fails with SPX_EMEM the same way, and the seam disarms after firing
+        L.spx_debug_fail_alloc(0)
+        h = C.c_void_p()
+        d = desc(m=m, n=n, nnz=nnz, rowptr64=rp.ctypes.data, col=c.ctypes.data, val=v.ctypes.data, kernel=2)
+        assert L.spx_plan_create(C.byref(h), C.byref(d)) == spx.EMEM
+    finally:
+        L.spx_debug_fail_alloc(-1)
+        L.spx_debug_set_free_mem(-1)
+
+
 def desc(**kw):
     d = spx.PlanDesc()
     spx.lib().spx_plan_desc_init(C.byref(d))
