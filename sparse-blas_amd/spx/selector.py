@@ -67,6 +67,12 @@ def distinct():
     return [s for s in range(NSCHED) if representative(s) == s]
 
 
+def single_partition():
+    """The distinct schedules with one partition (kernels 1-3) and kernel 4:
+    on one GPU more stream partitions only split the same CUs."""
+    return [s for s in distinct() if s >= 72 or s % 6 == 0]
+
+
 def long_row_threshold(n):
     return max(4096, 4 * ((n + 2047) // 2048))
 
@@ -230,13 +236,16 @@ def autotune(make_plan, x_dev, y_dev, candidates, warmup=2, reps=5):
     return min(times, key=times.get), times
 
 
-def exhaustive(A, x=None, warmup=2, reps=5):
-    """Time every distinct schedule on A (ms per execute, HIP events); the
-    copies of a kernel-4 schedule share its time."""
+def exhaustive(A, x=None, warmup=2, reps=5, only=None):
+    """Time every distinct schedule on A (or those in `only`; the rest stay
+    NaN) in ms per execute from HIP events; the copies of a kernel-4 schedule
+    share its time.  Large matrices get fewer repetitions."""
     x = W.vector(A["n"], 17) if x is None else x
     xd, yd = DeviceBuffer.from_array(x), DeviceBuffer(max(A["m"], 1) * 8)
+    if A["nnz"] > 2e7:
+        warmup, reps = 1, 3
     t = np.full(NSCHED, np.nan)
-    for s in distinct():
+    for s in (distinct() if only is None else only):
         try:
             P = Plan(A["rowptr"], A["col"], A["val"], n=A["n"], **schedule(s))
         except Exception as e:  # a schedule that cannot be built stays untimed (NaN)
@@ -249,7 +258,7 @@ def exhaustive(A, x=None, warmup=2, reps=5):
     return t
 
 
-def training_matrix(seed, nmin=1e4, nmax=2e6, nnz_cap=8e6):
+def training_matrix(seed, nmin=1e4, nmax=6e6, nnz_cap=4e7):
     """A seeded matrix for the selector's data: n log-uniform in [nmin, nmax],
     mean row length log-uniform in [2, 200] (capped by nnz_cap), law uniform
     over Zipf, lognormal, banded (bandwidth 3..101), block-diagonal with
@@ -294,14 +303,17 @@ def _zipf_kmax(z, mean, n):
     return hi
 
 
-def collect(seeds, out, **kw):
-    """Features and exhaustive timings of training matrices; saved as npz."""
+def collect(seeds, out, schedules="all", **kw):
+    """Features and exhaustive timings of training matrices; saved as npz
+    after every matrix.  schedules: "all" distinct ones, or "single" (one
+    partition and kernel 4)."""
+    only = None if schedules == "all" else single_partition()
     F, T, meta = [], [], []
     for i, seed in enumerate(seeds):
         t0 = time.time()
         A, law = training_matrix(seed, **kw)
         f = features(A)
-        t = exhaustive(A)
+        t = exhaustive(A, only=only)
         F.append(f)
         T.append(t)
         meta.append((seed, A["m"], A["nnz"], law))
@@ -325,8 +337,9 @@ def main(argv=None):
     c = sub.add_parser("collect")
     c.add_argument("out")
     c.add_argument("--seeds", default="0:8")
-    c.add_argument("--nmax", type=float, default=2e6)
-    c.add_argument("--nnz-cap", type=float, default=8e6)
+    c.add_argument("--nmax", type=float, default=6e6)
+    c.add_argument("--nnz-cap", type=float, default=4e7)
+    c.add_argument("--schedules", choices=["all", "single"], default="all")
     t = sub.add_parser("train")
     t.add_argument("data", nargs="+")
     t.add_argument("--model", default=MODEL_PATH)
@@ -336,7 +349,7 @@ def main(argv=None):
     e.add_argument("--model", default=MODEL_PATH)
     a = ap.parse_args(argv)
     if a.cmd == "collect":
-        collect(_seeds(a.seeds), a.out, nmax=a.nmax, nnz_cap=a.nnz_cap)
+        collect(_seeds(a.seeds), a.out, schedules=a.schedules, nmax=a.nmax, nnz_cap=a.nnz_cap)
     elif a.cmd == "train":
         X, Tm = [], []
         for path in a.data:
