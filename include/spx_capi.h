@@ -84,8 +84,9 @@ typedef struct spx_plan_desc {
  * (the plan copies them; the caller keeps ownership). */
 #define SPX_FLAG_DEVICE_PTRS 1
 /* desc.flags: kernel-4 FP64 plans run spx_plan_execute_multi as fused passes
- * of up to 16 columns (A read once per pass; the lanes of each wave split into
- * one group per column); without it every column is a separate execute. */
+ * of up to 64 columns (A read once per pass and held in registers while the
+ * columns of X go by; each column of Y is bitwise what a separate execute
+ * gives); without it every column is a separate execute. */
 #define SPX_FLAG_FUSED_SPMM 2
 
 typedef struct spx_plan *spx_plan_t;
@@ -107,8 +108,9 @@ int spx_plan_execute(spx_plan_t p, double alpha, const double *x, double beta,
 
 /* Multi-RHS: Y(:,j) = alpha*A*X(:,j) + beta*Y(:,j) for j < k, X (n-by-k) and
  * Y (m-by-k) column-major with leading dimensions ldx >= n, ldy >= m (see
- * SPX_FLAG_FUSED_SPMM).  The fused passes keep a plan-owned scratch of
- * n * 32 doubles (allocated on first use). */
+ * SPX_FLAG_FUSED_SPMM).  A fused pass covers up to 64 columns and reads A
+ * once; with long rows it keeps a plan-owned scratch of one double per long-row
+ * slice and column (allocated on first use). */
 int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
                            int64_t ldx, double beta, double *Y, int64_t ldy,
                            void *stream);

@@ -285,48 +285,7 @@ __global__ void k_reset(int *q, int n) {
   if (i < n) q[i] = 0;
 }
 
-// ---- long rows: fold the per-column-block partial sums -----------------------
-// The segmented engine sweeps the long rows by column blocks and leaves one
-// partial sum per (column block, row) slice; lseg[lsp[r] .. lsp[r+1]) lists
-// row r's slices in column order.  One workgroup per long row adds them in a
-// fixed tree order, then y[row] = alpha * sum + beta * y[row] (`assign`), or
-// y[row] += alpha * sum when the regular pass has already written beta * y0
-// for the row (it is empty there).
-__device__ inline void long_reduce(const double *__restrict__ partial, const int32_t *__restrict__ lsp,
-                                   const int32_t *__restrict__ lseg, const int32_t *__restrict__ rows, int r,
-                                   double alpha, double beta, int assign, double *__restrict__ y) {
-  // Up to U slices per thread per round: all their slot indices are loaded
-  // first, then all the partials, so a row of up to U*kBlock slices costs two
-  // dependent memory round trips.  A thread adds its slices in slot order.
-  constexpr int U = 16;
-  __shared__ double s_w[kBlock / kWave];
-  const int tid = threadIdx.x;
-  const int j0 = lsp[r], j1 = lsp[r + 1];
-  double s = 0.0;
-  for (int j = j0 + tid; j < j1; j += U * kBlock) {
-    int q[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) q[k] = j + k * kBlock < j1 ? lseg[j + k * kBlock] : -1;
-    double v[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) v[k] = q[k] >= 0 ? partial[q[k]] : 0.0;
-#pragma unroll
-    for (int k = 0; k < U; k++) s += v[k];
-  }
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-  if (tid % kWave == 0) s_w[tid / kWave] = s;
-  __syncthreads();
-  if (tid == 0) {
-    double t = 0.0;
-    for (int w = 0; w < kBlock / kWave; w++) t += s_w[w];
-    const int row = rows[r];
-    if (!assign)
-      y[row] += alpha * t;
-    else
-      y[row] = beta == 0.0 ? alpha * t : alpha * t + beta * y[row];
-  }
-}
+// ---- long rows: the fold of the per-column-block partial sums (long_reduce, seg_device.h)
 
 __global__ __launch_bounds__(kBlock) void k_long_reduce(const double *__restrict__ partial, int nlong,
                                                         const int32_t *__restrict__ lsp,

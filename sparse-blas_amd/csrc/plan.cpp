@@ -121,9 +121,8 @@ struct spx_plan {
   SegUnit *d_sunits = nullptr;
   int32_t *d_sxr = nullptr;  // per-XCD unit ranges, balanced by work: all, staged, unstaged (3 x 9)
   uint8_t *d_sstart = nullptr;
-  double *d_xr = nullptr;     // multi-RHS scratch: the pass's X columns, row-major
-  double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials
-  int64_t nxr = 0, npartk = 0;
+  double *d_partk = nullptr;  // multi-RHS scratch: long-row slice partials, column-major
+  int64_t npartk = 0;
   bool fused_spmm = false;
   uint32_t *d_scol12 = nullptr;  // staged units' 12-bit column offsets, 3 words per 8 elements
   int32_t *d_scol32 = nullptr;
@@ -153,7 +152,7 @@ struct spx_plan {
     for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_vunits, (void *)d_munits,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_tunits, (void *)d_empty, (void *)d_queue,
-                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_xr, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
+                    (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
                     (void *)d_scol12, (void *)d_scol32, (void *)d_sval, (void *)d_sval32})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
@@ -1447,18 +1446,9 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
   if (p->fused_spmm && p->kernel == SPX_KERNEL_SEG && p->precision == SPX_PREC_F64) {
     // fused passes of up to kSpmmPass columns: A is read once per pass
     hipStream_t st = (hipStream_t)stream;
-    const int K = spmm_width(std::min(k, kSpmmPass));
-    if (p->nxr < (int64_t)p->n * K || (p->nlong > 0 && p->npartk < p->nlseg * K))
-      HIP_TRY(hipDeviceSynchronize());  // earlier passes may still read the scratch being replaced
-    if (p->nxr < (int64_t)p->n * K) {
-      if (p->d_xr) HIP_TRY(hipFree(p->d_xr));
-      p->d_xr = nullptr;
-      p->nxr = 0;
-      int rc = p->alloc(&p->d_xr, (int64_t)p->n * K);
-      if (rc) return rc;
-      p->nxr = (int64_t)p->n * K;
-    }
+    const int K = std::min(k, kSpmmPass);
     if (p->nlong > 0 && p->npartk < p->nlseg * K) {
+      HIP_TRY(hipDeviceSynchronize());  // earlier passes may still read the scratch being replaced
       if (p->d_partk) HIP_TRY(hipFree(p->d_partk));
       p->d_partk = nullptr;
       p->npartk = 0;
@@ -1477,9 +1467,7 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
       a.val = p->d_sval;
       a.X = X + (size_t)j0 * ldx;
       a.ldx = ldx;
-      a.n = p->n;
       a.kk = std::min(kSpmmPass, k - j0);
-      a.XR = p->d_xr;
       a.alpha = alpha;
       a.beta = beta;
       a.Y = Y + (size_t)j0 * ldy;
@@ -1487,6 +1475,7 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
       a.empty = p->d_empty;
       a.nempty = p->nempty;
       a.partial = p->d_partk;
+      a.pstride = p->nlseg;
       a.nlong = p->nlong;
       a.lsp = p->d_lsp;
       a.lseg = p->d_lseg;

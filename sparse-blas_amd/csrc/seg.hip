@@ -33,11 +33,12 @@ __device__ inline void seg_emit(const SegArgs<V, X> &a, bool part, int o, double
   if (part)
     a.partial[o] = s;
   else
-    a.y[o] = a.beta == 0.0 ? a.alpha * s : a.alpha * s + a.beta * a.y[o];
+    a.y[o] = axpby(a.alpha, s, a.beta, a.beta == 0.0 ? 0.0 : a.y[o]);
 }
 
 template <typename V, typename X, bool STAGE>
 __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_seg(SegArgs<V, X> a) {
+#pragma clang fp contract(off)  // the sums round as written (run_sums, axpby): the SpMM kernel matches them
   constexpr int LE = 8;
   constexpr int CH = kSegBlock * LE;
   constexpr int NW = kSegBlock / kWave;
@@ -123,39 +124,27 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
             vv[i] = (V)0;
           }
         }
-        // every lane gathers (index 0 past the unit's end, always in range) so
-        // the 8 gathers issue back to back instead of each behind a branch
+        // All 8 slots are gathered and multiplied: a lane's slots past the
+        // unit's end hold value 0 and column (offset) 0, always in range, and
+        // feed only running sums that are never stored (that lane's last
+        // segment ends at the unit's end, so the lane carries nothing on).
+        double xg[LE];
         if (stg) {
 #pragma unroll
-          for (int i = 0; i < LE; i++) {
-            const int cc = i < nv ? ci[i] : 0;  // in [0, cwin) for real nonzeros
-            const double t = prod(vv[i], s_x + lead, cc);
-            p[i] = i < nv ? t : 0.0;
-          }
+          for (int i = 0; i < LE; i++) xg[i] = s_x[lead + ci[i]];
         } else {
 #pragma unroll
-          for (int i = 0; i < LE; i++) {
-            const int cc = i < nv ? ci[i] : 0;
-            const double t = prod(vv[i], a.x, cc);
-            p[i] = i < nv ? t : 0.0;
-          }
+          for (int i = 0; i < LE; i++) xg[i] = a.x[ci[i]];
         }
+        run_sums(vv, xg, fb, p);
       }
-      // Running sums that restart at every start bit: p[i] becomes the sum of
-      // the thread's products from the last start bit at or before i (or from
-      // its first product) through i.  Piece 0 is the run before the first
-      // inner start bit; the last piece's sum is p[nv-1].
+      // Piece 0 is the run before the first inner start bit; the sum the
+      // lane leaves open is its 8th slot's (a lane holding fewer than 8
+      // elements ends the unit and closes its last segment).
       const uint32_t inner = fb & ~1u;
       const int f1 = inner ? __builtin_ctz(inner) : nv;
-#pragma unroll
-      for (int i = 1; i < LE; i++)
-        if (!((fb >> i) & 1)) p[i] = p[i - 1] + p[i];
-      double last = p[0];
-#pragma unroll
-      for (int i = 1; i < LE; i++)
-        if (i < nv) last = p[i];
       const int has_close = nv > 0 && (inner != 0 || tclosed);
-      const double run = nv > 0 && !tclosed ? last : 0.0;
+      const double run = nv > 0 && !tclosed ? p[LE - 1] : 0.0;
       const int cnt = __builtin_popcount(fb);
       // segmented inclusive scan c(t) = run(t) + (has_close(t) ? 0 : c(t-1)),
       // and an inclusive scan of the start-bit counts, across the wave

@@ -41,12 +41,8 @@ __device__ inline void load8i(const int32_t *p, int64_t B, int L, int lane, int3
   o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = u.w; o[4] = w.x; o[5] = w.y; o[6] = w.z; o[7] = w.w;
 }
 
-// The 12-bit column offsets of the 8 elements from ea (a multiple of 8):
-// three consecutive 32-bit words, unpacked to int.
-__device__ inline void load8u12(const uint32_t *col12, int ea, int32_t *o) {
-  const uint32_t *p = col12 + (int64_t)(ea >> 3) * 3;
-  const uint32_t d0 = __builtin_nontemporal_load(p), d1 = __builtin_nontemporal_load(p + 1),
-                 d2 = __builtin_nontemporal_load(p + 2);
+// The 8 12-bit column offsets packed in three 32-bit words, unpacked to int.
+__device__ inline void unpack12(uint32_t d0, uint32_t d1, uint32_t d2, int32_t *o) {
   o[0] = (int32_t)(d0 & 0xfffu);
   o[1] = (int32_t)((d0 >> 12) & 0xfffu);
   o[2] = (int32_t)(__builtin_amdgcn_alignbit(d1, d0, 24) & 0xfffu);
@@ -55,6 +51,34 @@ __device__ inline void load8u12(const uint32_t *col12, int ea, int32_t *o) {
   o[5] = (int32_t)(__builtin_amdgcn_alignbit(d2, d1, 28) & 0xfffu);
   o[6] = (int32_t)((d2 >> 8) & 0xfffu);
   o[7] = (int32_t)(d2 >> 20);
+}
+
+// The 12-bit column offsets of the 8 elements from ea (a multiple of 8):
+// three consecutive 32-bit words.
+__device__ inline void load8u12(const uint32_t *col12, int ea, int32_t *o) {
+  const uint32_t *p = col12 + (int64_t)(ea >> 3) * 3;
+  unpack12(__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1), __builtin_nontemporal_load(p + 2), o);
+}
+
+// A lane's running sums over its 8 elements, restarting at each start bit
+// (bit i of f): p[i] = v[i]*xg[i] where element i starts a segment, else
+// p[i-1] + v[i]*xg[i], as one fused multiply-add for FP64 values (the
+// precision form adds its split product).  Written out so that every kernel
+// that forms these sums rounds them alike.
+__device__ inline void run_sums(const double *v, const double *xg, uint32_t f, double *p) {
+  p[0] = v[0] * xg[0];
+#pragma unroll
+  for (int i = 1; i < 8; i++) p[i] = __builtin_fma(v[i], xg[i], (f >> i) & 1 ? 0.0 : p[i - 1]);
+}
+__device__ inline void run_sums(const float *v, const double *xg, uint32_t f, double *p) {
+  p[0] = prod(v[0], xg[0]);
+#pragma unroll
+  for (int i = 1; i < 8; i++) p[i] = ((f >> i) & 1 ? 0.0 : p[i - 1]) + prod(v[i], xg[i]);
+}
+
+// The stored result: alpha*s, or alpha*s + beta*y0 rounded once after beta*y0.
+__device__ inline double axpby(double alpha, double s, double beta, double y0) {
+  return beta == 0.0 ? alpha * s : __builtin_fma(alpha, s, beta * y0);
 }
 
 // One step of a wave64 inclusive scan with DPP (data-parallel primitives:
@@ -98,6 +122,49 @@ __device__ inline void scan_step_v(double (&v)[NC], int &sg) {
     if (!sg) v[j] += __hiloint2double(hi, lo);
   }
   sg |= su;
+}
+
+// Long rows: fold the per-column-block partial sums.
+// The segmented engine sweeps the long rows by column blocks and leaves one
+// partial sum per (column block, row) slice; lseg[lsp[r] .. lsp[r+1]) lists
+// row r's slices in column order.  One workgroup per long row adds them in a
+// fixed tree order, then y[row] = alpha * sum + beta * y[row] (`assign`), or
+// y[row] += alpha * sum when the regular pass has already written beta * y0
+// for the row (it is empty there).
+__device__ inline void long_reduce(const double *__restrict__ partial, const int32_t *__restrict__ lsp,
+                                   const int32_t *__restrict__ lseg, const int32_t *__restrict__ rows, int r,
+                                   double alpha, double beta, int assign, double *__restrict__ y) {
+  // Up to U slices per thread per round: all their slot indices are loaded
+  // first, then all the partials, so a row of up to U*kBlock slices costs two
+  // dependent memory round trips.  A thread adds its slices in slot order.
+  constexpr int U = 16;
+  __shared__ double s_w[kBlock / kWave];
+  const int tid = threadIdx.x;
+  const int j0 = lsp[r], j1 = lsp[r + 1];
+  double s = 0.0;
+  for (int j = j0 + tid; j < j1; j += U * kBlock) {
+    int q[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) q[k] = j + k * kBlock < j1 ? lseg[j + k * kBlock] : -1;
+    double v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) v[k] = q[k] >= 0 ? partial[q[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < U; k++) s += v[k];
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (tid % kWave == 0) s_w[tid / kWave] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; w++) t += s_w[w];
+    const int row = rows[r];
+    if (!assign)
+      y[row] = __builtin_fma(alpha, t, y[row]);
+    else
+      y[row] = axpby(alpha, t, beta, y[row]);
+  }
 }
 
 }  // namespace spx

@@ -170,7 +170,7 @@ struct SegArgs {
 
 // One multi-RHS pass (spmm.hip): columns of X and Y starting at the given
 // pointers, kk <= kSpmmPass of them.
-constexpr int kSpmmPass = 16;  // right-hand sides per fused SpMM pass (A read once per pass)
+constexpr int kSpmmPass = 64;  // right-hand sides per fused SpMM pass (A read once per pass)
 
 struct SpmmArgs {
   const SegUnit *units;
@@ -182,14 +182,14 @@ struct SpmmArgs {
   const double *val;
   const double *X;  // the pass's first column
   int64_t ldx;
-  int n, kk;   // columns of A; columns in this pass, 1..kSpmmPass
-  double *XR;  // n * spmm_width(kk) doubles: the pass's columns of X, row-major
+  int kk;  // columns in this pass, 1..kSpmmPass
   double alpha, beta;
   double *Y;  // the pass's first column
   int64_t ldy;
   const int32_t *empty;  // rows of A without nonzeros (long rows excluded)
   int64_t nempty;
-  double *partial;  // slices * spmm_width(kk) doubles
+  double *partial;  // slice o of column j at partial[j * pstride + o]
+  int64_t pstride;
   int nlong;
   const int32_t *lsp, *lseg, *lrows;
 };
@@ -278,9 +278,7 @@ hipError_t launch_unit_window(const int32_t *col, int32_t *f, int stride, int of
 // MUnit table of a partition from its coordinates (units + 1 entries).
 hipError_t launch_merge_units(const Coord *coords, int64_t units, MUnit *out, hipStream_t st);
 
-// Multi-RHS SpMM pass on the segmented engine's layout (spmm.hip);
-// spmm_width(kk) = kk rounded up to a power of two (a pass's partial stride).
-int spmm_width(int kk);
+// Multi-RHS SpMM pass on the segmented engine's layout (spmm.hip).
 hipError_t launch_spmm(const SpmmArgs &a, hipStream_t st);
 
 // Segmented engine: `stage` = some unit stages x in LDS (SEG_STAGE).

@@ -1,9 +1,10 @@
 """Multi-RHS SpMM (spx_plan_execute_multi): Y = alpha*A*X + beta*Y with X, Y
 column-major, against the CPU oracle column by column.  Kernel-4 plans with
-FLAG_FUSED_SPMM run the fused passes (up to 16 columns per read of A); the
-others loop the single-vector path.  Fused sums are formed in a fixed order
-(within a group of 8 nonzeros, then across groups), so small-integer inputs
-give the oracle's bits."""
+FLAG_FUSED_SPMM run the fused passes (up to 64 columns per read of A, the
+matrix held in registers while the columns go by); the others loop the
+single-vector path.  Fused sums are formed in the single-vector engine's
+order, so every fused column is bitwise the single-vector result, and
+small-integer inputs give the oracle's bits."""
 import numpy as np
 import pytest
 
@@ -30,9 +31,10 @@ def run_multi(A, k, alpha, beta, pad=(0, 0), seed=1, vals=W.VAL_UNIFORM, **kw):
     return Y[:, :m], np.stack(refs), P
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 8, 17, 32, 37])
-@pytest.mark.parametrize("kc", [dict(kernel=4, flags=spx.FLAG_FUSED_SPMM), dict(kernel=4), dict(kernel=2)],
-                         ids=["k4-fused", "k4-loop", "k2-loop"])
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 17, 32, 37, 70])
+@pytest.mark.parametrize("kc", [dict(kernel=4, flags=spx.FLAG_FUSED_SPMM), dict(kernel=4, items=8, flags=spx.FLAG_FUSED_SPMM),
+                                dict(kernel=4), dict(kernel=2)],
+                         ids=["k4-fused", "k4-fused-8chunk-units", "k4-loop", "k2-loop"])
 def test_spmm_matches_oracle(k, kc):
     A = W.zipf(6000, 1.2, 3000, seed=31)
     Y, R, P = run_multi(A, k, 0.75, -0.5, pad=(5, 3), long_row=200, **kc)
@@ -76,3 +78,22 @@ def test_spmm_reproducible_and_equal_to_repeated_calls():
     Y1, _, P = run_multi(A, 8, 1.0, 0.0, kernel=4, flags=spx.FLAG_FUSED_SPMM)
     Y2, _, _ = run_multi(A, 8, 1.0, 0.0, kernel=4, flags=spx.FLAG_FUSED_SPMM)
     assert np.array_equal(Y1, Y2)
+
+
+@pytest.mark.parametrize("items", [2, 8])
+def test_spmm_fused_columns_bitwise_equal_single_vector_execute(items):
+    # the fused pass adds in the single-vector engine's order: same bits per
+    # column, long rows included
+    A = W.zipf(8000, 1.2, 4000, seed=12)
+    m, n, k = A["m"], A["n"], 9
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=4, items=items, long_row=150,
+                 flags=spx.FLAG_FUSED_SPMM)
+    assert P.info()["nlong"] > 0
+    X = W.vector(n * k, 21).reshape(k, n)
+    Y0 = W.vector(m * k, 22).reshape(k, m)
+    xd, yd = spx.DeviceBuffer.from_array(X.ravel()), spx.DeviceBuffer.from_array(Y0.ravel())
+    P.execute_multi(k, xd, n, yd, m, 1.25, 0.5)
+    spx.synchronize()
+    Yf = yd.download(np.float64, m * k).reshape(k, m)
+    for j in range(k):
+        assert np.array_equal(Yf[j], P(X[j], 1.25, 0.5, y=Y0[j])), j
