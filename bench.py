@@ -61,9 +61,19 @@ def run_precision(a):
     log(f"workload {a.workload}: m={m} n={n} nnz={nnz} mean_row={A['mean_row']:.2f} max_row={A['max_row']}")
     x = W.vector(n, 31)
     xd, yd = spx.DeviceBuffer.from_array(x), spx.DeviceBuffer(m * 8)
-    import oracle  # the checker
-    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    # the CPU baseline leg: the reference's host loop (the oracle) on 1 thread,
+    # timed; its y is also the checker's reference
+    import oracle
+    reps = []
+    for _ in range(3):
+        t = time.perf_counter()
+        ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+        reps.append((time.perf_counter() - t) * 1e3)
+    cpu_ms = float(np.median(reps))
     _, mag = oracle.spmv_dot2(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    cpu = {"value": round(W.bytes_model(m, n, nnz, 8) / (cpu_ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": 1,
+           "kind": "port", "sample": f"whole matrix, FP64, median of 3 ({cpu_ms:.1f} ms/SpMV); main.cu:350-356 "
+                                     "loop restated in C"}
     res = {}
     for prec, tag, sv in ((spx.PREC_F64, "fp64", 8), (spx.PREC_F32, "fp32", 4)):
         P = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=a.kernel, items=a.items, parts=a.parts,
@@ -88,6 +98,7 @@ def run_precision(a):
         "dtype": "f32 storage, f64 accumulation", "data": "synthetic",
         "config": {"workload": f"{a.workload}-like CSR {m}x{n}, nnz={nnz}", "kernel": a.kernel},
         "fp64": res["fp64"], "fp32": f32, "speedup_fp32_vs_fp64": round(res["fp64"]["ms_per_step"] / f32["ms_per_step"], 3),
+        "kernel_ratio_fp32_over_fp64": round(f32["kernel_ms"] / res["fp64"]["kernel_ms"], 3), "cpu_baseline": cpu,
     }), flush=True)
 
 
@@ -231,12 +242,18 @@ def run_batch(a):
         g.close()
         bp.close()
 
-    import oracle  # the checker: sampled members against the CPU oracle
+    # the CPU baseline leg: the reference's host loop (the oracle) on 1 thread
+    # over 16 sampled members, timed; its y is also the checker's reference
+    import oracle
     rng = np.random.default_rng(0)
     errs = []
+    cpu_s, cpu_n = 0.0, 0
     for i in sorted(rng.choice(len(mats), min(16, len(mats)), replace=False)):
         A = mats[i]
+        t = time.perf_counter()
         ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x[off[i]:off[i + 1]], 1.0, 0.0)
+        cpu_s += time.perf_counter() - t
+        cpu_n += 1
         yi = out["fp32"][1][off[i]:off[i + 1]]
         errs.append(float(np.linalg.norm(yi - ref) / np.linalg.norm(ref)))
     B32 = sum(W.bytes_model(A["m"], A["n"], A["nnz"], 4) for A in mats)
@@ -257,6 +274,9 @@ def run_batch(a):
         "fp32_GBs": round(B32 / (ms32 * 1e-3) / 1e9, 1), "fp64_batched_GBs": round(B64 / (ms64 * 1e-3) / 1e9, 1),
         "plain_GBs": round(B64 / (plain_ms * 1e-3) / 1e9, 1),
         "fp32_normwise_rel_err_max": max(errs), "fp32_normwise_rel_err_median": float(np.median(errs)),
+        "cpu_baseline": {"value": round(cpu_n / cpu_s, 1), "unit": "matrices/s", "cores": 1, "kind": "port",
+                         "sample": f"{cpu_n} sampled members, FP64, one pass each ({cpu_s * 1e3:.1f} ms in all); "
+                                   "main.cu:350-356 loop restated in C"},
     }
     print(json.dumps(res), flush=True)
 
@@ -384,7 +404,8 @@ def main():
         Bu = W.bytes_model(Au["m"], Au["n"], Au["nnz"], 8, beta != 0.0)
         uniform = {"workload": f"same generator, every row's columns uniform at random (nnz={Au['nnz']})",
                    "ms_per_step": round(wu, 5), "kernel_ms": round(ku, 5), "GBs": round(Bu / (wu * 1e-3) / 1e9, 2),
-                   "kernel_frac_of_hbm_peak": round(Bu / (ku * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                   "kernel_frac_of_hbm_peak": round(Bu / (ku * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "column_bands": Pu.info()["bands"]}
         log(f"uniform columns: wall {wu:.4f} ms/step, kernel {ku:.4f} ms")
         Pu.close()
         del Au

@@ -124,7 +124,7 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X,
  * [21] 1 when staged and unstaged units run as two concurrent kernels
  * [22] host microseconds spent copying the input's columns and values to
  * the device during the build (kernels 1-3) [23] host microseconds of the
- * whole build.
+ * whole build [24] column bands of kernel 4's regular rows (0: row units).
  * Returns the count. */
 int spx_plan_info(spx_plan_t p, int64_t *out, int cap);
 

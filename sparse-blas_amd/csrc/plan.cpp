@@ -134,6 +134,12 @@ struct spx_plan {
   int64_t nstaged_units = 0;         // units that stage their x window (any order)
   int64_t nlseg = 0;                 // long-row slices
   bool sstage = false;
+  // kernel 4, column bands: the regular rows cut at band edges; slice (b, r)
+  // sums into partial[nlseg + k] (band-major, rows ascending within a band)
+  int nbands = 0;
+  int64_t nbslots = 0;            // band slices
+  uint32_t *d_bmask = nullptr;    // per row: bit b = a slice in band b; bit 31 = long row
+  int32_t *d_bbase = nullptr;     // [ceil(m/64) * 32]: slot of the first slice of band b at or after row 64w
   hipEvent_t lfork = nullptr, ldone = nullptr;
 
   struct Launch {
@@ -153,7 +159,8 @@ struct spx_plan {
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_tunits, (void *)d_empty, (void *)d_queue,
                     (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
-                    (void *)d_scol12, (void *)d_scol32, (void *)d_sval, (void *)d_sval32})
+                    (void *)d_scol12, (void *)d_scol32, (void *)d_sval, (void *)d_sval32, (void *)d_bmask,
+                    (void *)d_bbase})
       if (p) (void)hipFree(p);
     for (auto s : streams) (void)hipStreamDestroy(s);
     for (auto e : done) (void)hipEventDestroy(e);
@@ -393,7 +400,7 @@ static int seg_upload(spx_plan *P, SegHost &H) {
   H.val.resize(total, 0.0);
   H.start.resize((size_t)(total >> 3), 0);
   // the kernel's preconditions, checked once on the host
-  const int64_t nslots = P->nlseg;
+  const int64_t nslots = P->nlseg + P->nbslots;
   for (const SegUnit &U : H.units) {
     bool ok = U.e0 % 8 == 0 && U.e0 < U.e1 && U.e1 <= H.pos && U.ns >= 1 && U.s0 >= 0 && U.o0 >= 0 &&
               (int64_t)U.o0 + U.ns <= ((U.flags & SEG_PARTIAL) ? nslots : (int64_t)P->m) &&
@@ -433,7 +440,8 @@ static int seg_upload(spx_plan *P, SegHost &H) {
       }
     }
     const double xbytes = 8.0;  // an x entry (FP64, or the FP32 hi/lo pair)
-    P->seg_split = nst > 0 && wu >= 0.1 * wall && el > 0 && span / el * xbytes > 4.0 * (1 << 20);
+    // column-band plans always split: their slices run without the x window
+    P->seg_split = nst > 0 && (P->nbands > 0 || (wu >= 0.1 * wall && el > 0 && span / el * xbytes > 4.0 * (1 << 20)));
     if (P->seg_split)
       std::stable_partition(H.units.begin(), H.units.end(),
                             [](const SegUnit &U) { return (U.flags & SEG_STAGE) != 0; });
@@ -662,6 +670,111 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   return build_csr(P, d, rp, long_rows, in);
 }
 
+// Column bands for kernel 4's regular rows.  A unit of rows whose columns
+// spread over more of x than one XCD's L2 holds gathers every entry from the
+// MALL or HBM, a whole cache line per 8-byte entry.  When most regular
+// nonzeros sit in rows wider than one band, the rows are cut at band edges
+// instead: band b's slices (rows ascending) form their own units, so the
+// units an XCD runs at a time gather from one band of x that stays in its
+// L2, and each slice's sum goes to a partial slot; k_band_reduce then adds a
+// row's slices in band order.  Bands are kBandCols columns (2 MiB of x) wide,
+// at most 31 per plan (wider when x is larger).
+constexpr int64_t kBandCols = 1 << 18;
+constexpr int kMaxBands = 31;
+
+// The number of bands for these regular rows, or 0 for the row-unit layout.
+static int band_count(int m, int n, const std::vector<int32_t> &rp, const int32_t *col) {
+  if ((int64_t)n < 2 * kBandCols) return 0;
+  int64_t wide = 0;
+  for (int r = 0; r < m; r++) {
+    const int64_t b = rp[r], e = rp[r + 1];
+    if (e - b < 2) continue;
+    int32_t lo = col[b], hi = col[b];
+    for (int64_t k = b + 1; k < e; k++) {
+      lo = std::min(lo, col[k]);
+      hi = std::max(hi, col[k]);
+    }
+    if ((int64_t)hi - lo >= kBandCols) wide += e - b;
+  }
+  if (2 * wide < (int64_t)rp[m]) return 0;
+  return (int)std::min<int64_t>(kMaxBands, ((int64_t)n + kBandCols - 1) / kBandCols);
+}
+
+// The regular rows as band slices in units (SEG_PARTIAL), plus the per-row
+// band masks and per-64-row slot bases for k_band_reduce.
+static int seg_bands(spx_plan *P, SegHost &H, int nb, const std::vector<int32_t> &rp, const int32_t *col,
+                     const double *val, const std::vector<int32_t> &long_rows, int64_t target) {
+  const int m = P->m;
+  const int64_t width = ((int64_t)P->n + nb - 1) / nb;
+  const int64_t nnz = rp[m];
+  std::vector<uint32_t> mask((size_t)m, 0);
+  for (int r : long_rows) mask[r] = 1u << 31;
+  std::vector<int64_t> elems(nb + 1, 0), slices(nb + 1, 0);
+  for (int r = 0; r < m; r++) {
+    uint32_t mk = 0;
+    for (int64_t k = rp[r]; k < rp[r + 1]; k++) {
+      const int b = (int)(col[k] / width);
+      mk |= 1u << b;
+      elems[b + 1]++;
+    }
+    for (uint32_t t = mk; t; t &= t - 1) slices[__builtin_ctz(t) + 1]++;
+    mask[r] |= mk;
+  }
+  for (int b = 0; b < nb; b++) {
+    elems[b + 1] += elems[b];
+    slices[b + 1] += slices[b];
+  }
+  const int64_t S = slices[nb];
+  if (P->nlseg + S >= INT32_MAX) return fail(SPX_ERANGE, "too many band slices");
+  // band-major copies of the columns and values, each slice's range, and
+  // the slot of each band's first slice at every 64th row
+  std::vector<int32_t> bcol((size_t)nnz);
+  std::vector<double> bval((size_t)nnz);
+  std::vector<int64_t> sb((size_t)S);
+  std::vector<int32_t> sl((size_t)S);
+  std::vector<int64_t> epos(elems.begin(), elems.end() - 1), spos(slices.begin(), slices.end() - 1);
+  const int64_t nw = ((int64_t)m + kWave - 1) / kWave;
+  std::vector<int32_t> base((size_t)(nw * (kMaxBands + 1)), 0);  // 32 per wave, unused bands 0
+  for (int r = 0; r < m; r++) {
+    if (r % kWave == 0)
+      for (int b = 0; b < nb; b++) base[(size_t)(r / kWave) * (kMaxBands + 1) + b] = (int32_t)(P->nlseg + spos[b]);
+    const uint32_t mk = mask[r] & ~(1u << 31);
+    for (uint32_t t = mk; t; t &= t - 1) {
+      const int b = __builtin_ctz(t);
+      sb[(size_t)spos[b]] = epos[b];
+    }
+    for (int64_t k = rp[r]; k < rp[r + 1]; k++) {
+      const int b = (int)(col[k] / width);
+      bcol[(size_t)epos[b]] = col[k];
+      bval[(size_t)epos[b]] = val[k];
+      epos[b]++;
+    }
+    for (uint32_t t = mk; t; t &= t - 1) {
+      const int b = __builtin_ctz(t);
+      sl[(size_t)spos[b]] = (int32_t)(epos[b] - sb[(size_t)spos[b]]);
+      spos[b]++;
+    }
+  }
+  for (int b = 0; b < nb; b++) {
+    const int64_t s0 = slices[b];
+    seg_append(
+        H, slices[b + 1] - s0,
+        [&](int64_t q, int64_t &e0, int64_t &e1, int32_t &o) {
+          e0 = sb[(size_t)(s0 + q)];
+          e1 = e0 + sl[(size_t)(s0 + q)];
+          o = (int32_t)(P->nlseg + s0 + q);
+        },
+        [](int64_t) { return false; }, bcol.data(), bval.data(), target, true, SEG_PARTIAL);
+    if (H.pos + 64 >= INT32_MAX) return fail(SPX_ERANGE, "padded nonzeros must stay below 2^31 per plan");
+  }
+  P->nbands = nb;
+  P->nbslots = S;
+  int rc;
+  if ((rc = P->upload(&P->d_bmask, mask.data(), m))) return rc;
+  if ((rc = P->upload(&P->d_bbase, base.data(), nw * (kMaxBands + 1)))) return rc;
+  return SPX_OK;
+}
+
 // Kernel 4: the unit table is built on the host (it needs each unit's column
 // window), then uploaded in the segmented layout.
 static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
@@ -749,7 +862,6 @@ static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
     if ((rc = P->upload(&P->d_lrows, long_rows.data(), P->nlong))) return rc;
     if ((rc = P->upload(&P->d_lsp, lsp.data(), P->nlong + 1))) return rc;
     if ((rc = P->upload(&P->d_lseg, lseg.data(), P->nlseg))) return rc;
-    if ((rc = P->alloc(&P->d_partial, P->nlseg))) return rc;
   }
 
   // ---- the regular rows join the long slices in one launch ----
@@ -766,16 +878,23 @@ static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
         empty.push_back(i);
     if (P->long_nnz + nnzr + 8 * ((int64_t)rows.size() + (int64_t)P->nblocks * P->nlong) + 64 >= INT32_MAX)
       return fail(SPX_ERANGE, "padded nonzeros must stay below 2^31 per plan");
-    seg_append(
-        H, (int64_t)rows.size(),
-        [&](int64_t i, int64_t &b, int64_t &e, int32_t &o) {
-          o = rows[i];
-          b = rp[o];
-          e = rp[o + 1];
-        },
-        [&](int64_t i) { return rows[i] != rows[i - 1] + 1; }, src_col, src_val, seg_target, true, 0);
     int rc;
+    const int bands = band_count(m, n, rp, src_col);
+    if (bands > 0) {
+      if ((rc = seg_bands(P, H, bands, rp, src_col, src_val, long_rows, seg_target))) return rc;
+      empty.clear();  // the band reduction writes every regular row
+    } else {
+      seg_append(
+          H, (int64_t)rows.size(),
+          [&](int64_t i, int64_t &b, int64_t &e, int32_t &o) {
+            o = rows[i];
+            b = rp[o];
+            e = rp[o + 1];
+          },
+          [&](int64_t i) { return rows[i] != rows[i - 1] + 1; }, src_col, src_val, seg_target, true, 0);
+    }
     if ((rc = seg_upload(P, H))) return rc;
+    if (P->nlseg + P->nbslots > 0 && (rc = P->alloc(&P->d_partial, P->nlseg + P->nbslots))) return rc;
     if (P->seg_split) {
       HIP_TRY(hipStreamCreateWithFlags(&P->lstream, hipStreamNonBlocking));
       HIP_TRY(hipEventCreateWithFlags(&P->lfork, hipEventDisableTiming));
@@ -1364,14 +1483,15 @@ static hipError_t launch_seg(spx_plan *P, int cls, const double *x, double alpha
   if (cnt <= 0) return hipSuccess;
   const int32_t *xr = P->d_sxr + cls * (kXcd + 1);
   const bool stage = cls == SEG_STAGED || (cls == SEG_ALL && P->sstage);
+  const bool wide_out = P->nbands > 0 && !stage;  // column-band slices: one or two elements each
   if (P->precision == SPX_PREC_F32) {
     SegArgs<float, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval32,
                                 x, alpha, beta, y, P->d_partial};
-    return launch_seg_f32(a, stage, st);
+    return launch_seg_f32(a, stage, wide_out, st);
   }
   SegArgs<double, double> a = {P->d_sunits, cnt, xr, P->d_sstart, P->d_scol12, P->d_scol32, P->d_sval,
                                x, alpha, beta, y, P->d_partial};
-  return launch_seg_f64(a, stage, st);
+  return launch_seg_f64(a, stage, wide_out, st);
 }
 
 // Enqueue one SpMV.  t0/t1 (optional) bracket the SpMV kernels proper (the
@@ -1402,6 +1522,9 @@ static int execute_impl(spx_plan *P, double alpha, const double *x, double beta,
     } else {
       HIP_TRY(launch_seg(P, SEG_ALL, x, alpha, beta, y, st));
     }
+    // column bands: the regular rows' slices, added per row in band order
+    if (P->nbands > 0) HIP_TRY(launch_band_reduce(P->d_bmask, P->d_bbase, P->nbands, P->d_partial, P->nlseg + P->nbslots, P->m, alpha,
+                                        beta, y, st));
     if (t1) HIP_TRY(hipEventRecord(t1, st));
     if (P->nlong > 0)
       HIP_TRY(launch_long_reduce(P->d_partial, P->nlong, P->d_lsp, P->d_lseg, P->d_lrows, alpha, beta, true, y, st));
@@ -1443,7 +1566,7 @@ int spx_plan_execute_multi(spx_plan_t p, int k, double alpha, const double *X, i
   if (!p || k < 0 || ldx < p->n || ldy < p->m) return fail(SPX_EARG, "bad multi-RHS arguments");
   if (p->m == 0 || k == 0) return SPX_OK;
   if (!X || !Y) return fail(SPX_EARG, "null X or Y");
-  if (p->fused_spmm && p->kernel == SPX_KERNEL_SEG && p->precision == SPX_PREC_F64) {
+  if (p->fused_spmm && p->kernel == SPX_KERNEL_SEG && p->precision == SPX_PREC_F64 && p->nbands == 0) {
     // fused passes of up to kSpmmPass columns: A is read once per pass
     hipStream_t st = (hipStream_t)stream;
     const int K = std::min(k, kSpmmPass);
@@ -1499,7 +1622,7 @@ int spx_plan_info(spx_plan_t p, int64_t *out, int cap) {
                  p->kernel == SPX_KERNEL_SEG ? p->nsunits : p->nunits, p->ncarry, p->ntiles,
                  p->bytes,  p->precision, p->queues,   p->nlong,  p->long_nnz, p->nblocks,
                  0,  p->long_th, p->nstaged_units, p->seg_split ? 1 : 0,
-                 (int64_t)(p->upload_ms * 1000.0), (int64_t)(p->build_ms * 1000.0)};
+                 (int64_t)(p->upload_ms * 1000.0), (int64_t)(p->build_ms * 1000.0), p->nbands};
   int c = std::min<int>(cap, (int)(sizeof(v) / sizeof(v[0])));
   for (int i = 0; i < c; i++) out[i] = v[i];
   return c;

@@ -28,15 +28,17 @@
 
 namespace spx {
 
+// Partials are read once, by the folds after the kernel: stored non-temporal,
+// so they do not evict the x entries that the units' gathers reuse from L2.
 template <typename V, typename X>
 __device__ inline void seg_emit(const SegArgs<V, X> &a, bool part, int o, double s) {
   if (part)
-    a.partial[o] = s;
+    __builtin_nontemporal_store(s, &a.partial[o]);
   else
     a.y[o] = axpby(a.alpha, s, a.beta, a.beta == 0.0 ? 0.0 : a.y[o]);
 }
 
-template <typename V, typename X, bool STAGE>
+template <typename V, typename X, bool STAGE, int NOUT>
 __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_seg(SegArgs<V, X> a) {
 #pragma clang fp contract(off)  // the sums round as written (run_sums, axpby): the SpMM kernel matches them
   constexpr int LE = 8;
@@ -46,10 +48,10 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
   __shared__ double s_wv[2][NW];  // per wave: scan value, closed flag, start-bit count
   __shared__ int s_ws[2][NW];
   __shared__ int s_wc[2][NW];
-  __shared__ double s_out[kSegOut];  // sums closed in the current chunk, by output
-  __shared__ uint8_t s_ok[kSegOut];  // which of them are set
+  __shared__ double s_out[NOUT];  // sums closed in the current chunk, by output
+  __shared__ uint8_t s_ok[NOUT];  // which of them are set
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-  for (int t = tid; t < kSegOut; t += kSegBlock) s_ok[t] = 0;
+  for (int t = tid; t < NOUT; t += kSegBlock) s_ok[t] = 0;
   __syncthreads();
   int par = 0;  // double-buffer slot of the wave totals, alternating per chunk
   SegWalk w = seg_walk(a.xr);
@@ -181,9 +183,9 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
       const int excl = cb + cn - cnt;  // output of the first segment begun by this thread
       // The sums closed in this chunk have outputs in [base - 1, totc): they
       // go to LDS and leave in one coalesced pass, unless the chunk closes
-      // more than kSegOut of them (then each is stored where it closes).
+      // more than NOUT of them (then each is stored where it closes).
       const int kb = base - 1;
-      const bool via_lds = totc - kb <= kSegOut;  // uniform
+      const bool via_lds = totc - kb <= NOUT;  // uniform
       auto put = [&](int k, double v) {
         if (via_lds) {
           s_out[k - kb] = v;
@@ -219,6 +221,73 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(5))) 
   }
 }
 
+// Kernel 4 with column bands: wave w owns rows 64w .. 64w+63.  Row r's
+// slice in band b sits at base[w * 32 + b] plus the number of the wave's
+// earlier rows with a slice in band b (a ballot and mbcnt), so for each band
+// the wave reads one contiguous run of partials.  Every band's load is
+// issued before the first is added (a lane without a slice in a band reads
+// the slot its next row's slice would take, within the same lines, and
+// drops it).  A row adds its slices in band order; long rows (mask
+// bit 31) are left to the long-row fold, rows without nonzeros get beta * y.
+// The waves are persistent (a grid-stride walk over the 64-row groups):
+// one short wave per group is bound by the rate at which waves launch.
+constexpr int kMaxBandsDev = 31;  // bands per plan at most (bit 31 of a row's mask marks a long row)
+
+__global__ __launch_bounds__(kBlock) void k_band_reduce(const uint32_t *__restrict__ mask,
+                                                        const int32_t *__restrict__ base,
+                                                        int nb, const double *__restrict__ partial, int last,
+                                                        int m, double alpha, double beta, double *__restrict__ y) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x % kWave;
+  const int nw = (m + kWave - 1) / kWave;
+  const int step = __builtin_amdgcn_readfirstlane((int)(gridDim.x * (blockDim.x / kWave)));
+  for (int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave)); w < nw;
+       w += step) {
+    const int64_t r = (int64_t)w * kWave + lane;
+    const uint32_t mk = r < m ? __builtin_nontemporal_load(&mask[r]) : 1u << 31;
+    // the wave's 32 slot bases in one go (128 bytes, 16-byte aligned)
+    int32_t bw[kMaxBandsDev + 1];
+    {
+      const vi4 *p4 = reinterpret_cast<const vi4 *>(base + (int64_t)w * (kMaxBandsDev + 1));
+#pragma unroll
+      for (int q = 0; q < (kMaxBandsDev + 1) / 4; q++) {
+        const vi4 t = p4[q];
+        bw[4 * q] = t.x;
+        bw[4 * q + 1] = t.y;
+        bw[4 * q + 2] = t.z;
+        bw[4 * q + 3] = t.w;
+      }
+    }
+    double v[kMaxBandsDev];
+#pragma unroll
+    for (int b = 0; b < kMaxBandsDev; b++) {
+      const uint32_t bit = (mk >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      const int below =
+          (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+      const int at = bw[b] + below;
+      v[b] = b < nb ? __builtin_nontemporal_load(&partial[at < last ? at : last]) : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < kMaxBandsDev; b++)
+      if ((mk >> b) & 1u) s += v[b];
+    if (r < m && !(mk >> 31))
+      y[r] = mk == 0 ? (beta == 0.0 ? 0.0 : beta * y[r]) : axpby(alpha, s, beta, beta == 0.0 ? 0.0 : y[r]);
+  }
+}
+
+hipError_t launch_band_reduce(const uint32_t *mask, const int32_t *base, int nb, const double *partial,
+                              int64_t nslots, int m, double alpha, double beta, double *y, hipStream_t st) {
+  if (m <= 0 || nb <= 0 || nslots <= 0) return hipSuccess;
+  if (nb > kMaxBandsDev) return hipErrorInvalidValue;
+  const int64_t waves = ((int64_t)m + kWave - 1) / kWave;
+  const int64_t blocks = std::min<int64_t>((waves * kWave + kBlock - 1) / kBlock, 2048);
+  hipLaunchKernelGGL(k_band_reduce, dim3((unsigned)blocks), dim3(kBlock), 0, st, mask, base, nb, partial,
+                     (int)(nslots - 1), m, alpha, beta, y);
+  return hipGetLastError();
+}
+
 template <typename K>
 static int seg_cap(K kern) {
   int nb = 0, dev = 0, cus = 0;
@@ -230,21 +299,31 @@ static int seg_cap(K kern) {
   return cap < kXcd ? kXcd : cap;
 }
 
-template <typename V, typename X, bool STAGE>
+template <typename V, typename X, bool STAGE, int NOUT>
 static hipError_t seg_launch(const SegArgs<V, X> &a, hipStream_t st) {
   if (a.nunits <= 0) return hipSuccess;
-  static const int cap = seg_cap(k_seg<V, X, STAGE>);
+  static const int cap = seg_cap(k_seg<V, X, STAGE, NOUT>);
   int g = direct_grid(a.nunits);
   if (g > cap) g = cap;
-  hipLaunchKernelGGL((k_seg<V, X, STAGE>), dim3(g), dim3(kSegBlock), 0, st, a);
+  hipLaunchKernelGGL((k_seg<V, X, STAGE, NOUT>), dim3(g), dim3(kSegBlock), 0, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st) {
-  return stage ? seg_launch<double, double, true>(a, st) : seg_launch<double, double, false>(a, st);
+// `wide_out`: units of one- or two-element segments (column-band slices)
+// close up to a whole chunk of sums, and gathering all of them in LDS
+// (kSegOutWide) beats storing each where it closes; only kernels without
+// the x window have the LDS for it.
+template <typename V>
+static hipError_t seg_launch_any(const SegArgs<V, double> &a, bool stage, bool wide_out, hipStream_t st) {
+  if (stage) return seg_launch<V, double, true, kSegOut>(a, st);
+  return wide_out ? seg_launch<V, double, false, kSegOutWide>(a, st) : seg_launch<V, double, false, kSegOut>(a, st);
 }
-hipError_t launch_seg_f32(const SegArgs<float, double> &a, bool stage, hipStream_t st) {
-  return stage ? seg_launch<float, double, true>(a, st) : seg_launch<float, double, false>(a, st);
+
+hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, bool wide_out, hipStream_t st) {
+  return seg_launch_any(a, stage, wide_out, st);
+}
+hipError_t launch_seg_f32(const SegArgs<float, double> &a, bool stage, bool wide_out, hipStream_t st) {
+  return seg_launch_any(a, stage, wide_out, st);
 }
 
 }  // namespace spx

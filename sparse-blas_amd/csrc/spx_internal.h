@@ -115,6 +115,7 @@ constexpr int kSegStage = 3328;  // most x entries a unit stages in LDS (26 KiB)
 static_assert(kSegStage <= 4096, "staged column offsets are 12-bit");
 constexpr int kSegBlock = 256;   // threads per segmented-engine workgroup (4 wave64s)
 constexpr int kSegOut = 520;     // sums per chunk gathered in LDS before they are stored
+constexpr int kSegOutWide = kSegBlock * 8 + 8;  // every sum a chunk can close
 
 // One workgroup unit of the segmented engine: padded elements [e0, e1)
 // (e0 a multiple of 8) holding segments s0 .. s0 + ns - 1 in order.  A
@@ -278,11 +279,16 @@ hipError_t launch_unit_window(const int32_t *col, int32_t *f, int stride, int of
 // MUnit table of a partition from its coordinates (units + 1 entries).
 hipError_t launch_merge_units(const Coord *coords, int64_t units, MUnit *out, hipStream_t st);
 
+// Kernel 4 with column bands: y for every regular row from its band slices
+// (seg.hip).
+hipError_t launch_band_reduce(const uint32_t *mask, const int32_t *base, int nbands, const double *partial,
+                              int64_t nslots, int m, double alpha, double beta, double *y, hipStream_t st);
+
 // Multi-RHS SpMM pass on the segmented engine's layout (spmm.hip).
 hipError_t launch_spmm(const SpmmArgs &a, hipStream_t st);
 
 // Segmented engine: `stage` = some unit stages x in LDS (SEG_STAGE).
-hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, hipStream_t st);
-hipError_t launch_seg_f32(const SegArgs<float, double> &a, bool stage, hipStream_t st);
+hipError_t launch_seg_f64(const SegArgs<double, double> &a, bool stage, bool wide_out, hipStream_t st);
+hipError_t launch_seg_f32(const SegArgs<float, double> &a, bool stage, bool wide_out, hipStream_t st);
 
 }  // namespace spx

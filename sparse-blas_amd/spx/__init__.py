@@ -235,7 +235,7 @@ class Plan:
         k = lib().spx_plan_info(self.h, out, 32)
         keys = ["m", "n", "nnz", "kernel", "mode", "parts", "sigma", "lanes", "items", "units",
                 "carries", "tiles", "bytes", "precision", "queues", "nlong", "long_nnz", "col_blocks",
-                "runs", "long_row", "staged_units", "split", "upload_us", "build_us"]
+                "runs", "long_row", "staged_units", "split", "upload_us", "build_us", "bands"]
         return {keys[i]: int(out[i]) for i in range(k)}
 
     def export(self, what):

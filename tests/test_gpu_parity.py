@@ -280,32 +280,103 @@ def c2():
     return A, x, ref
 
 
+def mixed_rows(m, n, wide_frac, seed, n_long=3, long_len=6000):
+    """Rows of 0..12 nonzeros: a fraction `wide_frac` at uniform columns over
+    [0, n), the rest within +-1024 of the scaled diagonal; plus `n_long` rows
+    of `long_len` uniform columns and some empty rows."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 13, m)
+    longs = rng.choice(m, n_long, replace=False)
+    k[longs] = long_len
+    wide = rng.random(m) < wide_frac
+    cols = []
+    for r in range(m):
+        if k[r] == 0:
+            cols.append(np.zeros(0, np.int64))
+        elif wide[r] or k[r] == long_len:
+            cols.append(np.sort(rng.choice(n, k[r], replace=False)))
+        else:
+            c = int(r * (n / m))
+            lo, hi = max(0, c - 1024), min(n, c + 1025)
+            cols.append(np.sort(rng.choice(np.arange(lo, hi), min(k[r], hi - lo), replace=False)))
+    rp = np.zeros(m + 1, np.int64)
+    rp[1:] = np.cumsum([len(c) for c in cols])
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.standard_normal(col.size)
+    return W.finish(rp, col, val, m, n)
+
+
+def check_any(A, x, y, alpha, beta, y0, precision):
+    if precision == spx.PREC_F64:
+        check_fp64(A, x, y, alpha, beta, y0 if beta != 0 else None)
+    else:  # FP32 values exactly, x as a (hi, lo) pair: within 1e-12 of the FP32-value loop
+        ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, alpha, beta,
+                                 y0 if beta != 0 else None)
+        assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("precision", [spx.PREC_F64, spx.PREC_F32])
 @pytest.mark.parametrize("items", [1, 4])
 def test_kernel4_split_into_two_concurrent_kernels(items, precision):
-    """A circuit-like matrix whose regular rows have uniform columns over
-    600,000: their units cannot stage x and gather over 4.8 MB, so kernel 4
-    runs them as a second kernel beside the staged long-row units."""
-    A = W.circuit(m=600_000, total=4_000_000, n_dense=20, window=0, seed=5)
+    """30 % of the regular rows at uniform columns over 600,000: their units
+    cannot stage x and gather over 4.8 MB, so kernel 4 runs them as a second
+    kernel beside the staged units (too few for column bands)."""
+    A = mixed_rows(200_000, 600_000, 0.3, seed=5)
     x = W.vector(A["n"], 23)
     y0 = W.vector(A["m"], 24)
     print(f"split case items={items} precision={precision} nnz={A['nnz']}", flush=True)
     P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=4, items=items, precision=precision)
     info = P.info()
-    assert info["split"] == 1 and 0 < info["staged_units"] < info["units"], info
+    assert info["split"] == 1 and info["bands"] == 0 and 0 < info["staged_units"] < info["units"], info
     for alpha, beta in [(1.0, 0.0), (0.5, -2.0)]:
-        y = P(x, alpha, beta, y0)
-        if precision == spx.PREC_F64:
-            check_fp64(A, x, y, alpha, beta, y0 if beta != 0 else None)
-        else:  # FP32 values exactly, x as a (hi, lo) pair: within 1e-12 of the FP32-value loop
-            ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, alpha, beta,
-                                     y0 if beta != 0 else None)
-            assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+        check_any(A, x, P(x, alpha, beta, y0), alpha, beta, y0, precision)
     # the same plan twice gives the same bits (no order depends on the two kernels' timing)
     assert np.array_equal(P(x, 0.5, -2.0, y0), P(x, 0.5, -2.0, y0))
     # and a banded matrix of the same size does not split
     Ab = W.circuit(m=600_000, total=4_000_000, n_dense=20, window=1024, seed=5)
     assert spx.Plan(Ab["rowptr"], Ab["col"], Ab["val"], n=Ab["n"], kernel=4, items=items).info()["split"] == 0
+
+
+@pytest.mark.parametrize("precision", [spx.PREC_F64, spx.PREC_F32])
+@pytest.mark.parametrize("case", ["circuit-uniform", "mixed-empty-long"])
+def test_kernel4_column_bands(case, precision):
+    """Regular rows mostly at uniform columns over more than two bands of x:
+    kernel 4 cuts them at band edges (262,144 columns), runs each band's
+    slices as units and adds a row's slices in band order."""
+    if case == "circuit-uniform":
+        A = W.circuit(m=600_000, total=4_000_000, n_dense=20, window=0, seed=5)
+    else:
+        A = mixed_rows(120_000, 1_000_000, 0.9, seed=17)
+    x = W.vector(A["n"], 25)
+    y0 = W.vector(A["m"], 26)
+    print(f"bands case {case} precision={precision} nnz={A['nnz']}", flush=True)
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=4, precision=precision)
+    info = P.info()
+    assert info["bands"] == -(-A["n"] // 262_144) and info["nlong"] > 0, info
+    for alpha, beta in [(1.0, 0.0), (0.5, -2.0), (-1.25, 0.75)]:
+        check_any(A, x, P(x, alpha, beta, y0), alpha, beta, y0, precision)
+    assert np.array_equal(P(x, 0.5, -2.0, y0), P(x, 0.5, -2.0, y0))
+    if precision == spx.PREC_F64:
+        # multi-RHS on a banded plan: each column as a separate execute
+        k = 3
+        X = W.vector(A["n"] * k, 27).reshape(k, A["n"])
+        xd, yd = spx.DeviceBuffer.from_array(X.ravel()), spx.DeviceBuffer.from_array(np.tile(y0, k))
+        P.execute_multi(k, xd, A["n"], yd, A["m"], 0.5, -2.0)
+        spx.synchronize()
+        Y = yd.download(np.float64, A["m"] * k).reshape(k, A["m"])
+        for j in range(k):
+            assert np.array_equal(Y[j], P(X[j], 0.5, -2.0, y0)), j
+
+
+def test_kernel4_bands_small_integers_bit_exact():
+    A = mixed_rows(60_000, 800_000, 0.8, seed=29, long_len=5000)
+    rng = np.random.default_rng(3)
+    A["val"] = rng.integers(-8, 9, A["nnz"]).astype(np.float64)
+    x = rng.integers(-8, 9, A["n"]).astype(np.float64)
+    y0 = rng.integers(-8, 9, A["m"]).astype(np.float64)
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=4)
+    assert P.info()["bands"] == 4
+    assert np.array_equal(P(x, 2.0, -1.0, y0), oracle.spmv(A["rowptr"], A["col"], A["val"], x, 2.0, -1.0, y0))
 
 
 @pytest.mark.parametrize("kc", [dict(kernel=2), dict(kernel=3), dict(kernel=2, parts=8),
