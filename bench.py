@@ -77,7 +77,8 @@ def run_precision(a):
     res = {}
     for prec, tag, sv in ((spx.PREC_F64, "fp64", 8), (spx.PREC_F32, "fp32", 4)):
         P = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=a.kernel, items=a.items, parts=a.parts,
-                     sigma=a.sigma, precision=prec)
+                     sigma=a.sigma, precision=prec, long_row=a.long_row)
+        log(f"{tag} plan: {P.info()}")
         tot, ker, wall = P.time(xd, yd, 1.0, 0.0, warmup=a.warmup, reps=a.steps, wall=True)
         P.execute(xd, yd)
         spx.synchronize()
@@ -96,7 +97,8 @@ def run_precision(a):
         "value": f32["GBs"], "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": f32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 storage, f64 accumulation", "data": "synthetic",
-        "config": {"workload": f"{a.workload}-like CSR {m}x{n}, nnz={nnz}", "kernel": a.kernel},
+        "config": {"workload": f"{a.workload}-like CSR {m}x{n}, nnz={nnz}", "kernel": a.kernel, "items": a.items,
+                   "long_row": a.long_row},
         "fp64": res["fp64"], "fp32": f32, "speedup_fp32_vs_fp64": round(res["fp64"]["ms_per_step"] / f32["ms_per_step"], 3),
         "kernel_ratio_fp32_over_fp64": round(f32["kernel_ms"] / res["fp64"]["kernel_ms"], 3), "cpu_baseline": cpu,
     }), flush=True)
@@ -311,6 +313,8 @@ def main():
                     help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain; "
                          "rail / asic: the precision form (FP32 storage) vs plain FP64; spmm: fused multi-RHS")
     ap.add_argument("--k", type=int, default=8, help="spmm: right-hand sides")
+    ap.add_argument("--long-row", type=int, default=0,
+                    help="rail / asic: long-row threshold (0: automatic, < 0: no long rows)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--autotune-top", type=int, default=4,
                     help="batch: schedules of the selector's shortlist the autotuner times")

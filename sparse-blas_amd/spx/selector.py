@@ -13,9 +13,10 @@ schedule on seeded matrices (laws: Zipf, lognormal, banded, block-diagonal
 with dense rows) and stores features and times; `train` fits the MLP on the
 GPU against soft targets that favour near-best schedules.
 
-    python -m spx.selector collect OUT.npz --seeds 0:240
-    python -m spx.selector train TRAIN.npz MODEL.npz
-    python -m spx.selector evaluate MODEL.npz HELDOUT.npz
+    python -m spx.selector collect OUT.npz --seeds 0:1000 --schedules single
+    python -m spx.selector merge TRAIN.npz PART1.npz PART2.npz ...
+    python -m spx.selector train TRAIN.npz --model MODEL.npz
+    python -m spx.selector evaluate HELDOUT.npz --model MODEL.npz --train TRAIN.npz
 """
 from __future__ import annotations
 
@@ -326,6 +327,18 @@ def collect(seeds, out, schedules="all", **kw):
     return out
 
 
+def merge(parts, out):
+    """Concatenate collected npz files (same fields) into one."""
+    keys = ("features", "times", "seeds", "m", "nnz", "law")
+    cols = {k: [] for k in keys}
+    for path in parts:
+        with np.load(path, allow_pickle=False) as f:
+            for k in keys:
+                cols[k].append(f[k])
+    np.savez(out, **{k: np.concatenate(v) for k, v in cols.items()})
+    return out
+
+
 def _seeds(spec):
     a, b = spec.split(":")
     return list(range(int(a), int(b)))
@@ -347,8 +360,14 @@ def main(argv=None):
     e = sub.add_parser("evaluate")
     e.add_argument("data")
     e.add_argument("--model", default=MODEL_PATH)
+    e.add_argument("--train", nargs="*", default=[], help="training sets: the best fixed schedule is chosen on them")
+    m = sub.add_parser("merge")
+    m.add_argument("out")
+    m.add_argument("parts", nargs="+")
     a = ap.parse_args(argv)
-    if a.cmd == "collect":
+    if a.cmd == "merge":
+        merge(a.parts, a.out)
+    elif a.cmd == "collect":
         collect(_seeds(a.seeds), a.out, schedules=a.schedules, nmax=a.nmax, nnz_cap=a.nnz_cap)
     elif a.cmd == "train":
         X, Tm = [], []
@@ -366,25 +385,45 @@ def main(argv=None):
     else:
         with np.load(a.data, allow_pickle=False) as f:
             X, Tm = f["features"], f["times"]
+            nnz = f["nnz"] if "nnz" in f.files else None
+        train_times = None
+        if a.train:
+            train_times = np.concatenate([np.load(p, allow_pickle=False)["times"] for p in a.train])
         net = MLP.load(a.model)
-        print(report(net, X, Tm), flush=True)
+        print(report(net, X, Tm, train_times=train_times, nnz=nnz), flush=True)
 
 
-def report(net, X, Tm, train_times=None):
-    """Regret of the selector, the default schedule and the best fixed
-    schedule (chosen on train_times, else on these times)."""
+def report(net, X, Tm, train_times=None, nnz=None):
+    """Regret of the selector against exhaustive search, beside the default
+    schedule and the best fixed schedule (chosen on train_times, else on these
+    times); how often a stream-partition variant (parts > 1, kernels 1-3) is
+    the fastest schedule, and what leaving them all out costs; with nnz, the
+    selector's mean regret per nnz decade."""
     ch = net.choose(X)
     ref = Tm if train_times is None else train_times
     fixed = int(np.nanargmin(np.nanmean(ref / np.nanmin(ref, axis=1, keepdims=True), axis=0)))
+    r = regret(Tm, ch)
+    best = np.nanargmin(Tm, axis=1)
+    multi = np.array([s < 72 and s % 6 != 0 for s in best])
+    single = [s for s in range(NSCHED) if s >= 72 or s % 6 == 0]
+    t1 = np.nanmin(Tm[:, single], axis=1)
     out = {
         "matrices": int(len(X)),
-        "selector_regret_mean": float(regret(Tm, ch).mean()),
-        "selector_regret_median": float(np.median(regret(Tm, ch))),
+        "selector_regret_mean": float(r.mean()),
+        "selector_regret_median": float(np.median(r)),
+        "selector_regret_p90": float(np.quantile(r, 0.9)),
+        "exact_best_fraction": float(np.mean(r <= 1e-12)),
         "default_regret_mean": float(regret(Tm, np.full(len(X), DEFAULT)).mean()),
         "fixed_best": int(fixed),
+        "fixed_best_schedule": schedule(fixed),
         "fixed_regret_mean": float(regret(Tm, np.full(len(X), fixed)).mean()),
-        "exact_best_fraction": float(np.mean(regret(Tm, ch) <= 1e-12)),
+        "multi_partition_best_fraction": float(multi.mean()),
+        "single_partition_only_regret_mean": float(np.mean(t1 / np.nanmin(Tm, axis=1) - 1.0)),
+        "best_kernel_share": {int(k): float(np.mean(1 + np.minimum(best, 95) // 24 == k)) for k in (1, 2, 3, 4)},
     }
+    if nnz is not None:
+        dec = np.floor(np.log10(np.maximum(np.asarray(nnz, np.float64), 1.0))).astype(int)
+        out["selector_regret_by_nnz_decade"] = {int(d): round(float(r[dec == d].mean()), 4) for d in np.unique(dec)}
     return out
 
 
