@@ -312,13 +312,17 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ pa
 }
 
 // ---- kernel 1: CSR-vector, L lanes per row --------------------------------------
-// Bell & Garland's CSR-vector for wave64: a group of L lanes walks a row.  Each
-// lane takes pairs of neighbouring nonzeros (one 16-byte value load, one
-// 8-byte column load) from the even index at or below the row start, so the
-// group covers 2L nonzeros per step, and a group holds R rows at once so
-// their loads are in flight together.  The group's sum is reduced with DPP
-// lane permutations.  A unit is G*R consecutive rows of one partition; the
-// partition's split rows go to its carry slots.  The matrix arrays are
+// Bell & Garland's CSR-vector for wave64: a group of L lanes sums a row, and
+// a group holds R rows at once.  A unit is G*R consecutive rows of one
+// partition.  When the unit has at most kBlock*kVecItems nonzeros (the usual
+// case: the lane count follows the mean row length), its nonzeros are loaded
+// with coalesced loads that need no row bounds, so they are requested at
+// once with the x window and the row bounds; the products go to LDS and
+// each row's L lanes add theirs from there.  A larger unit's lanes load their
+// rows directly: pairs of neighbouring nonzeros (one 16-byte value load, one
+// 8-byte column load) from the even index at or below the row start, 2L
+// nonzeros per step.  The group's sum is reduced with DPP lane permutations.
+// The partition's split rows go to its carry slots.  The matrix arrays are
 // padded by two entries, so a pair load past the last nonzero is readable.
 
 typedef double vd2 __attribute__((ext_vector_type(2)));
@@ -367,7 +371,10 @@ __device__ inline double group_sum(double s) {
 template <typename V, int L, int R>
 __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
   constexpr int G = kBlock / L;
-  __shared__ __attribute__((aligned(16))) double s_x[kSegStage + 2];
+  constexpr int PCAP = kBlock * kVecItems;
+  constexpr int XS = (kSegStage + 2) > PCAP ? (kSegStage + 2) : PCAP;
+  // the x window, then (after every gather) the unit's products
+  __shared__ __attribute__((aligned(16))) double s_x[XS];
   __shared__ int s_slot;
   const int tid = threadIdx.x, g = tid / L, l = tid % L, lane = tid % kWave;
   int64_t iter = 0;
@@ -379,13 +386,17 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
     u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
     dv = desc_prefetch(a.vunits + a.unit_begin, u, a.unit_count, lane);
     const int r_lo = desc_field(dcur, 0), r_hi = desc_field(dcur, 1), s0 = desc_field(dcur, 2),
-              e1 = desc_field(dcur, 3), cmin = desc_field(dcur, 4), cwin = desc_field(dcur, 5);
+              e1 = desc_field(dcur, 3), cmin = desc_field(dcur, 4), cwin = desc_field(dcur, 5),
+              z0 = desc_field(dcur, 6), z1 = desc_field(dcur, 7);
+    const int nzu = z1 - z0;
     const bool stg = cwin > 0;
+    // via: the unit's products are formed with coalesced loads and staged
+    // in LDS, where each row's lanes sum them (the loads do not wait for
+    // the row bounds); else each row's lanes load their own nonzeros
+    const bool via = nzu <= PCAP;
     int lead = 0;
-    if (stg) {
-      __syncthreads();  // the previous unit's gathers from s_x are done
-      lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
-    }
+    if (stg || via) __syncthreads();  // the previous unit is done with s_x
+    if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
     // row bounds, clamped to the partition; fp bit 0: the row begins in the
     // partition, bit 1: it ends in it
     int b[R], e[R], fp[R];
@@ -402,51 +413,88 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
         fp[k] = (rb >= s0 ? 1 : 0) | (re <= e1 ? 2 : 0);
       }
     }
-    if (stg) {
-      stage_wait();
-      __syncthreads();  // every wave's part of the window is in LDS
-    }
-    // first step for all R rows at once: pair (b >> 1) + l covers
-    // elements 2p, 2p+1; products outside [b, e) are dropped and their
-    // gathers read the window's (or x's) first entry.  The steps are
-    // compiled twice, gathering from LDS or from x, so neither becomes a
-    // generic (flat) load.
     double sum[R];
-    auto steps = [&](auto gx) {
-      V v0[R], v1[R];
-      vi2 c[R];
+    if (via) {
+      int ci[kVecItems];
+      V vv[kVecItems];
+      __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+      for (int i = 0; i < kVecItems; i++) {
+        const int k = tid + i * kBlock;
+        ci[i] = k < nzu ? __builtin_nontemporal_load(a.col + z0 + k) : 0;
+        vv[i] = k < nzu ? __builtin_nontemporal_load(a.val + z0 + k) : (V)0;
+      }
+      __builtin_amdgcn_s_setprio(0);
+      double pv[kVecItems];
+      if (stg) {
+        stage_wait();
+        __syncthreads();  // the window is in LDS
+#pragma unroll
+        for (int i = 0; i < kVecItems; i++)
+          pv[i] = prod(vv[i], s_x[lead + (tid + i * kBlock < nzu ? ci[i] - cmin : 0)]);
+        __syncthreads();  // every gather is done: the products overwrite the window
+      } else {
+#pragma unroll
+        for (int i = 0; i < kVecItems; i++) pv[i] = prod(vv[i], a.x[ci[i]]);
+      }
+#pragma unroll
+      for (int i = 0; i < kVecItems; i++) {
+        const int k = tid + i * kBlock;
+        if (k < nzu) s_x[k] = pv[i];
+      }
+      __syncthreads();  // the unit's products are in LDS
 #pragma unroll
       for (int k = 0; k < R; k++) {
-        const int p = (b[k] >> 1) + l;
-        const int q = 2 * p < e[k] ? p : (b[k] >> 1);
-        Pair<V>::load(a.val, 2 * (int64_t)q, v0[k], v1[k]);
-        c[k] = __builtin_nontemporal_load(reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)q));
+        double t = 0.0;
+        for (int j = b[k] - z0 + l; j < e[k] - z0; j += L) t += s_x[j];
+        sum[k] = t;
       }
+    } else {
+      if (stg) {
+        stage_wait();
+        __syncthreads();  // every wave's part of the window is in LDS
+      }
+      // first step for all R rows at once: pair (b >> 1) + l covers
+      // elements 2p, 2p+1; products outside [b, e) are dropped and their
+      // gathers read the window's (or x's) first entry.  The steps are
+      // compiled twice, gathering from LDS or from x, so neither becomes a
+      // generic (flat) load.
+      auto steps = [&](auto gx) {
+        V v0[R], v1[R];
+        vi2 c[R];
 #pragma unroll
-      for (int k = 0; k < R; k++) {
-        const int j = 2 * ((b[k] >> 1) + l);
-        const bool ok0 = j >= b[k] && j < e[k], ok1 = j + 1 >= b[k] && j + 1 < e[k];
-        const double t0 = prod(v0[k], gx(c[k].x, ok0));
-        const double t1 = prod(v1[k], gx(c[k].y, ok1));
-        sum[k] = (ok0 ? t0 : 0.0) + (ok1 ? t1 : 0.0);
-      }
-    // rows longer than the first step
+        for (int k = 0; k < R; k++) {
+          const int p = (b[k] >> 1) + l;
+          const int q = 2 * p < e[k] ? p : (b[k] >> 1);
+          Pair<V>::load(a.val, 2 * (int64_t)q, v0[k], v1[k]);
+          c[k] = __builtin_nontemporal_load(reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)q));
+        }
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-      for (int p = (b[k] >> 1) + l + L; 2 * p < e[k]; p += L) {
-        V v0, v1;
-        Pair<V>::load(a.val, 2 * (int64_t)p, v0, v1);
-        const vi2 c = *reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)p);
-        const bool ok1 = 2 * p + 1 < e[k];
-        const double t0 = prod(v0, gx(c.x, true)), t1 = prod(v1, gx(c.y, ok1));
-        sum[k] += t0 + (ok1 ? t1 : 0.0);
-      }
+        for (int k = 0; k < R; k++) {
+          const int j = 2 * ((b[k] >> 1) + l);
+          const bool ok0 = j >= b[k] && j < e[k], ok1 = j + 1 >= b[k] && j + 1 < e[k];
+          const double t0 = prod(v0[k], gx(c[k].x, ok0));
+          const double t1 = prod(v1[k], gx(c[k].y, ok1));
+          sum[k] = (ok0 ? t0 : 0.0) + (ok1 ? t1 : 0.0);
+        }
+        // rows longer than the first step
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+          for (int p = (b[k] >> 1) + l + L; 2 * p < e[k]; p += L) {
+            V v0, v1;
+            Pair<V>::load(a.val, 2 * (int64_t)p, v0, v1);
+            const vi2 c = *reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)p);
+            const bool ok1 = 2 * p + 1 < e[k];
+            const double t0 = prod(v0, gx(c.x, true)), t1 = prod(v1, gx(c.y, ok1));
+            sum[k] += t0 + (ok1 ? t1 : 0.0);
+          }
+        }
+      };
+      if (stg)
+        steps([&](int c, bool ok) -> double { return s_x[lead + (ok ? c - cmin : 0)]; });
+      else
+        steps([&](int c, bool ok) -> double { return a.x[ok ? c : 0]; });
     }
-    };
-    if (stg)
-      steps([&](int c, bool ok) -> double { return s_x[lead + (ok ? c - cmin : 0)]; });
-    else
-      steps([&](int c, bool ok) -> double { return a.x[ok ? c : 0]; });
 #pragma unroll
     for (int k = 0; k < R; k++) {
       const double t = group_sum<L>(sum[k]);
@@ -469,16 +517,20 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
 // ---- kernel 2: merge path ------------------------------------------------------
 // Merrill & Garland (SC'16): the merge of row ends and nonzero indices is cut
 // into equal pieces of kBlock*IPT items, one per unit (MUnit: the merge-path
-// coordinates at both ends, found at plan time).  A workgroup stages the
-// unit's row ends and its products val*x[col] in LDS with coalesced loads;
-// each thread merges IPT consecutive items from LDS; a block-wide segmented
-// scan (DPP steps across each wave, wave totals through LDS) joins the row
-// sums that cross threads.  The unit's row sums are gathered back in LDS and
-// leave in one coalesced pass.  A row cut by a unit boundary leaves pieces in
-// the units' carry slots (2u: the piece continuing a row begun earlier,
-// 2u+1: the piece open at the unit's end), tagged with kSlotFirst /
-// kSlotFinal, and k_merge_fix adds them.  Every slot is rewritten each
-// time, so no slot state carries over between calls.
+// coordinates at both ends, found at plan time), so every unit does the same
+// work however the rows are shaped.  Inside a unit the row ends are staged in
+// LDS and turned into start bits (a row end inside the unit starts the next
+// nonempty row's run of nonzeros).  The products val*x[col] are formed with
+// coalesced loads (x gathered from the LDS window when the unit's columns fit
+// one) and transposed through LDS so that each thread owns IPT consecutive
+// nonzeros; running sums restart at the start bits, a block-wide segmented
+// scan (DPP steps across each wave, wave totals through LDS) joins the runs
+// that cross threads, and the scanned values go back to LDS, where each row
+// reads its sum at its last nonzero and leaves in one coalesced pass.  A row
+// cut by a unit boundary leaves pieces in the units' carry slots (2u: the
+// piece continuing a row begun earlier, 2u+1: the piece open at the unit's
+// end), tagged with kSlotFirst / kSlotFinal, and k_finish adds them.  Every
+// slot is rewritten each time, so no slot state carries over between calls.
 
 template <int CTRL, int ROWS, int BANKS>
 __device__ inline void scan_step_vs(double &v, int &sg) {
@@ -493,17 +545,20 @@ __device__ inline void scan_step_vs(double &v, int &sg) {
 
 template <typename V, int IPT>
 __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
+  static_assert(IPT >= 1 && IPT <= 32, "at most 32 nonzeros per thread");
   constexpr int TW = kBlock * IPT;
   constexpr int NW = kBlock / kWave;
-  // the x window, then (after every gather) the unit's products, then its row sums
-  constexpr int XS = (kSegStage + 2) > TW ? (kSegStage + 2) : TW;
+  // the x window, then (after every gather) the unit's products, then their
+  // scan; product k sits at k + k/8 (one pad per 8), so a thread's IPT
+  // consecutive products and its neighbours' fall in different LDS banks
+  constexpr int XS = (kSegStage + 2) > TW + TW / 8 ? (kSegStage + 2) : TW + TW / 8;
   __shared__ __attribute__((aligned(16))) double s_x[XS];
-  __shared__ int32_t s_re[TW];  // row ends rowptr[r0 + k + 1]
+  __shared__ int32_t s_re[TW + 1];          // rowptr[r0 + k], k = 0..rows
+  __shared__ uint32_t s_fl[TW / 32 + 2];    // start bits, one per nonzero of the unit
   __shared__ double s_wv[NW];
   __shared__ int s_ws[NW];
-  __shared__ int s_rb;  // rowptr[r0]
   __shared__ int s_slot;
-  double *const s_prod = s_x;
+  double *const s_p = s_x;
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
   int64_t iter = 0;
   int64_t u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
@@ -521,8 +576,8 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
     // the x window, the row ends and the unit's nonzeros are all requested
     // before the first wait
     if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
-    if (tid == 0) s_rb = a.rowptr[i0];
-    for (int k = tid; k < nrows; k += kBlock) s_re[k] = a.rowptr[i0 + k + 1];
+    for (int k = tid; k <= nrows; k += kBlock) s_re[k] = a.rowptr[i0 + k];
+    for (int w = tid; w < TW / 32 + 2; w += kBlock) s_fl[w] = 0u;
     int ci[IPT];
     V vv[IPT];
     __builtin_amdgcn_s_setprio(2);
@@ -533,79 +588,44 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
       vv[r] = k < nnzs ? __builtin_nontemporal_load(a.val + j0 + k) : (V)0;
     }
     __builtin_amdgcn_s_setprio(0);
+    if (stg) stage_wait();
+    __syncthreads();  // the window, the row ends and the cleared start bits are in LDS
+    // a row end at unit position p < nnzs: nonzero p starts a row
+    for (int k = tid; k < nrows; k += kBlock) {
+      const int p = s_re[k + 1] - j0;
+      if (p >= 0 && p < nnzs) atomicOr(&s_fl[p >> 5], 1u << (p & 31));
+    }
     double pv[IPT];
     if (stg) {
-      stage_wait();
-      __syncthreads();  // the window is in LDS
 #pragma unroll
       for (int r = 0; r < IPT; r++) pv[r] = prod(vv[r], s_x[lead + (tid + r * kBlock < nnzs ? ci[r] - cmin : 0)]);
-      __syncthreads();  // every gather is done: the products overwrite the window
     } else {
 #pragma unroll
       for (int r = 0; r < IPT; r++) pv[r] = prod(vv[r], a.x[ci[r]]);
     }
+    __syncthreads();  // every gather is done: the products overwrite the window
 #pragma unroll
     for (int r = 0; r < IPT; r++) {
       const int k = tid + r * kBlock;
-      if (k < nnzs) s_prod[k] = pv[r];
+      if (k < nnzs) s_p[k + (k >> 3)] = pv[r];
     }
-    __syncthreads();
+    __syncthreads();  // products and start bits are in LDS
 
-    // this thread's merge items [d, dend): binary search of the diagonal
-    const int total = nrows + nnzs;
-    const int d = tid * IPT < total ? tid * IPT : total;
-    const int dend = d + IPT < total ? d + IPT : total;
-    int lo = d - nnzs > 0 ? d - nnzs : 0, hi = d < nrows ? d : nrows;
-    while (lo < hi) {
-      const int piv = (lo + hi) >> 1;
-      if (s_re[piv] <= j0 + (d - piv - 1))
-        lo = piv + 1;
-      else
-        hi = piv;
-    }
-    const int row0 = lo, jt0 = d - lo;
-    // the thread's next IPT row ends and products, read together; the walk
-    // then runs in registers (k-th step: at most k of each consumed)
-    int re[IPT];
-    double pr[IPT];
+    // this thread's nonzeros [q0, q0 + IPT) of the unit, and their start bits
+    const int q0 = tid * IPT;
+    double p[IPT];
 #pragma unroll
-    for (int k = 0; k < IPT; k++) {
-      re[k] = row0 + k < nrows ? s_re[row0 + k] : INT32_MAX;
-      pr[k] = jt0 + k < nnzs ? s_prod[jt0 + k] : 0.0;
-    }
-    int ir = 0, jr = 0;
-    double run = 0.0;
-    double out[IPT];
-    uint32_t em = 0;  // bit k: item d + k ends a row
+    for (int k = 0; k < IPT; k++) p[k] = q0 + k < nnzs ? s_p[q0 + k + ((q0 + k) >> 3)] : 0.0;
+    const uint64_t w2 = (uint64_t)s_fl[q0 >> 5] | ((uint64_t)s_fl[(q0 >> 5) + 1] << 32);
+    const uint32_t fb = (uint32_t)(w2 >> (q0 & 31)) & (IPT == 32 ? 0xFFFFFFFFu : ((1u << IPT) - 1u));
+    // running sums that restart at every start bit
 #pragma unroll
-    for (int k = 0; k < IPT; k++) {
-      out[k] = 0.0;
-      int rk = re[0];
-      double pk = pr[0];
-#pragma unroll
-      for (int i = 1; i < IPT; i++) {
-        if (i <= k) {
-          rk = ir == i ? re[i] : rk;
-          pk = jr == i ? pr[i] : pk;
-        }
-      }
-      if (d + k < dend) {
-        // a row end precedes nonzero jt when rowptr[row + 1] <= jt
-        if (jt0 + jr >= nnzs || rk <= j0 + jt0 + jr) {
-          out[k] = run;
-          em |= 1u << k;
-          run = 0.0;
-          ir++;
-        } else {
-          run += pk;
-          jr++;
-        }
-      }
-    }
+    for (int k = 1; k < IPT; k++)
+      if (!((fb >> k) & 1u)) p[k] = p[k - 1] + p[k];
 
-    // segmented inclusive scan c(t) = run(t) + (t ends a row ? 0 : c(t-1))
-    double v = run;
-    int sg = em != 0;
+    // segmented inclusive scan c(t) = run(t) + (t holds a start bit ? 0 : c(t-1))
+    double v = p[IPT - 1];
+    int sg = fb != 0u;
     scan_step_vs<0x111, 0xf, 0xf>(v, sg);  // row_shr:1
     scan_step_vs<0x112, 0xf, 0xf>(v, sg);  // row_shr:2
     scan_step_vs<0x114, 0xf, 0xe>(v, sg);  // row_shr:4
@@ -616,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
       s_wv[wid] = v;
       s_ws[wid] = sg;
     }
-    __syncthreads();  // also: every thread is done reading s_prod
+    __syncthreads();
     double prev = 0.0;
 #pragma unroll
     for (int w = 0; w < NW; w++)
@@ -624,25 +644,21 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
     const double c_incl = v + (sg ? 0.0 : prev);
     double carry_in = __shfl_up(c_incl, 1, kWave);
     if (lane == 0) carry_in = prev;
-
-    // the unit's row sums, by local row, into LDS
-    {
-      int rr = row0;
-      double cin = carry_in;
+    // the run open at the thread's start continues the previous thread's
+    const int first = fb ? __builtin_ctz(fb) : IPT;
 #pragma unroll
-      for (int k = 0; k < IPT; k++) {
-        if ((em >> k) & 1u) {
-          s_prod[rr] = out[k] + cin;
-          cin = 0.0;
-          rr++;
-        }
-      }
+    for (int k = 0; k < IPT; k++) {
+      if (k < first) p[k] += carry_in;
+      if (q0 + k < nnzs) s_p[q0 + k + ((q0 + k) >> 3)] = p[k];
     }
-    __syncthreads();
-    const int rb = s_rb;
+    __syncthreads();  // the scanned values are in LDS
+
+    // row k's sum is the scanned value at its last nonzero in the unit
+    const int rb = s_re[0];
     const bool head_split = nrows > 0 && rb < j0;  // row r0 began in an earlier unit and ends here
     for (int k = tid; k < nrows; k += kBlock) {
-      const double sv = s_prod[k];
+      const int b0 = s_re[k] - j0, e = s_re[k + 1] - j0;
+      const double sv = e > (b0 > 0 ? b0 : 0) ? s_p[(e - 1) + ((e - 1) >> 3)] : 0.0;
       if (k == 0 && head_split) {
         a.crow[2 * gu] = (int32_t)((uint32_t)i0 | kSlotFinal);
         a.cval[2 * gu] = sv;
@@ -654,19 +670,20 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
       // nonzeros after the unit's last row end belong to row r1, still open
       int32_t hc = -1, tc = -1;
       double hv = 0.0, tv = 0.0;
+      const double last = nnzs > 0 ? s_p[(nnzs - 1) + ((nnzs - 1) >> 3)] : 0.0;
       if (nrows == 0) {
         if (nnzs > 0) {
           if (rb >= j0) {
             tc = i0 | kSlotFirst;  // row r0 = r1 begins in this unit
-            tv = c_incl;
+            tv = last;
           } else {
             hc = i0;  // a middle piece of a row spanning this unit
-            hv = c_incl;
+            hv = last;
           }
         }
-      } else if (j1 > s_re[nrows - 1]) {
+      } else if (j1 > s_re[nrows]) {
         tc = r1 | kSlotFirst;
-        tv = c_incl;
+        tv = last;
       }
       if (!head_split) {
         a.crow[2 * gu] = hc;
@@ -730,7 +747,7 @@ __device__ inline void scan_step_c(int &cn) {
   cn += __builtin_amdgcn_update_dpp(0, cn, CTRL, ROWS, BANKS, false);
 }
 
-template <typename V, int MAXS>
+template <typename V, int MAXS, bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V> a) {
   constexpr int NW = kBlock / kWave;
   __shared__ __attribute__((aligned(16))) double s_x[kSegStage + 2];
@@ -744,7 +761,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V> a) {
   __shared__ int s_kb[2];    // the chunk's first row (row sums gathered from it on)
   __shared__ int s_slot;
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-  const int sigma = a.sigma;
+  // EXACT: the tile width is MAXS, so the element loops need no bounds
+  const int sigma = EXACT ? MAXS : a.sigma;
   for (int t = tid; t < kTileOut; t += kBlock) s_ok[t] = 0;
   int par = 0;
   int64_t iter = 0;
@@ -883,31 +901,48 @@ __global__ __launch_bounds__(kBlock) void k_tile(KArgs<V> a) {
       const int kb = s_kb[par];
       const bool via_lds = !anyd && totc + 1 <= kTileOut;
       const uint32_t ends = nv > 0 ? ((fb >> 1) | ((uint32_t)tclosed << (nv - 1))) & vmask : 0u;
+      // the local rows this lane can close: [lo_lr, lo_lr + its start bits]
+      // (clean tiles); none of them a carry piece: every sum goes to LDS
+      const int lo_lr = R0 + excl - f00;
+      const int hi_lr = lo_lr + cnt;
+      const bool plain = via_lds && !dirty && !(!fs && lrs >= lo_lr && lrs <= hi_lr) && !(split_head && lo_lr <= 0) &&
+                         !(split_tail && hi_lr >= last_lr);
+      if (plain) {
 #pragma unroll
-      for (int k = 0; k < MAXS; k++) {
-        if ((ends >> k) & 1u) {
-          const double val = k < f1 ? p[k] + carry_in : p[k];
-          const int lr = dirty ? tile_row_search(a.rowptr, row_lo, s0, nz, R0, (int)(tpn & 0x7FFFFFFFu) + 1, lpos + k)
-                               : R0 + excl + __builtin_popcount(fb & ((2u << k) - 1u)) - f00;
-          const int r = row_lo + lr;
-          // the pieces cut by the unit's start or the partition's bounds go
-          // to carry slots, the rows complete here to y
-          if (lr == lrs && !fs) {
-            const bool fin = !(split_tail && lr == last_lr);
-            a.crow[2 * (int64_t)gu] = (int32_t)((uint32_t)r | (fin ? kSlotFinal : 0u));
-            a.cval[2 * (int64_t)gu] = val;
-          } else if (split_head && lr == 0) {
-            const bool fin = !(split_tail && lr == last_lr);
-            a.crow[2 * (int64_t)gu] = (int32_t)((uint32_t)r | (fin ? kSlotFinal : 0u));
-            a.cval[2 * (int64_t)gu] = val;
-          } else if (split_tail && lr == last_lr) {
-            a.crow[2 * (int64_t)gu + 1] = r | kSlotFirst;
-            a.cval[2 * (int64_t)gu + 1] = val;
-          } else if (via_lds) {
-            s_out[lr - kb] = val;
-            s_ok[lr - kb] = 1;
-          } else {
-            a.y[r] = y_update(a.alpha, a.beta, val, a.y, r);
+        for (int k = 0; k < MAXS; k++) {
+          if ((ends >> k) & 1u) {
+            const int t = lo_lr + __builtin_popcount(fb & ((2u << k) - 1u)) - kb;
+            s_out[t] = k < f1 ? p[k] + carry_in : p[k];
+            s_ok[t] = 1;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < MAXS; k++) {
+          if ((ends >> k) & 1u) {
+            const double val = k < f1 ? p[k] + carry_in : p[k];
+            const int lr = dirty ? tile_row_search(a.rowptr, row_lo, s0, nz, R0, (int)(tpn & 0x7FFFFFFFu) + 1, lpos + k)
+                                 : R0 + excl + __builtin_popcount(fb & ((2u << k) - 1u)) - f00;
+            const int r = row_lo + lr;
+            // the pieces cut by the unit's start or the partition's bounds go
+            // to carry slots, the rows complete here to y
+            if (lr == lrs && !fs) {
+              const bool fin = !(split_tail && lr == last_lr);
+              a.crow[2 * (int64_t)gu] = (int32_t)((uint32_t)r | (fin ? kSlotFinal : 0u));
+              a.cval[2 * (int64_t)gu] = val;
+            } else if (split_head && lr == 0) {
+              const bool fin = !(split_tail && lr == last_lr);
+              a.crow[2 * (int64_t)gu] = (int32_t)((uint32_t)r | (fin ? kSlotFinal : 0u));
+              a.cval[2 * (int64_t)gu] = val;
+            } else if (split_tail && lr == last_lr) {
+              a.crow[2 * (int64_t)gu + 1] = r | kSlotFirst;
+              a.cval[2 * (int64_t)gu + 1] = val;
+            } else if (via_lds) {
+              s_out[lr - kb] = val;
+              s_ok[lr - kb] = 1;
+            } else {
+              a.y[r] = y_update(a.alpha, a.beta, val, a.y, r);
+            }
           }
         }
       }
@@ -1128,12 +1163,26 @@ hipError_t launch_merge_f32(const KArgs<float> &a, int items, int grid, hipStrea
 }
 template <typename V>
 static hipError_t tile_dispatch(const KArgs<V> &a, int grid, hipStream_t st) {
-  if (a.sigma <= 8)
-    hipLaunchKernelGGL((k_tile<V, 8>), dim3(launch_grid(grid, RESIDENT((k_tile<V, 8>)))), dim3(kBlock), 0, st, a);
-  else if (a.sigma <= 16)
-    hipLaunchKernelGGL((k_tile<V, 16>), dim3(launch_grid(grid, RESIDENT((k_tile<V, 16>)))), dim3(kBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_tile<V, 32>), dim3(launch_grid(grid, RESIDENT((k_tile<V, 32>)))), dim3(kBlock), 0, st, a);
+#define SPX_TILE(S, E) \
+  hipLaunchKernelGGL((k_tile<V, S, E>), dim3(launch_grid(grid, RESIDENT((k_tile<V, S, E>)))), dim3(kBlock), 0, st, a)
+  // the widths the automatic choice gives short rows run with exact loops
+  switch (a.sigma) {
+    case 4: SPX_TILE(4, true); break;
+    case 5: SPX_TILE(5, true); break;
+    case 6: SPX_TILE(6, true); break;
+    case 7: SPX_TILE(7, true); break;
+    case 8: SPX_TILE(8, true); break;
+    case 16: SPX_TILE(16, true); break;
+    case 32: SPX_TILE(32, true); break;
+    default:
+      if (a.sigma <= 8)
+        SPX_TILE(8, false);
+      else if (a.sigma <= 16)
+        SPX_TILE(16, false);
+      else
+        SPX_TILE(32, false);
+  }
+#undef SPX_TILE
   return hipGetLastError();
 }
 hipError_t launch_tile_f64(const KArgs<double> &a, int grid, hipStream_t st) {

@@ -1255,9 +1255,8 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
         v.r_hi = (int32_t)std::min<int64_t>(q.row_hi, v.r_lo + G);
         v.s = (int32_t)q.s;
         v.e1 = (int32_t)(q.e + 1);
-        v.cmin = std::max<int32_t>(rp[v.r_lo], v.s);
-        v.cwin = std::min<int32_t>(rp[v.r_hi], v.e1);
-        v.pad0 = v.pad1 = 0;
+        v.cmin = v.z0 = std::max<int32_t>(rp[v.r_lo], v.s);
+        v.cwin = v.z1 = std::min<int32_t>(rp[v.r_hi], v.e1);
       }
     if ((rc = P->upload(&P->d_vunits, vu.data(), P->nunits))) return rc;
     HIP_TRY(launch_unit_window(P->d_col, reinterpret_cast<int32_t *>(P->d_vunits), 8, 4, P->nunits, 0));

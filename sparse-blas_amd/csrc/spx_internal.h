@@ -74,8 +74,11 @@ struct MUnit {
 struct VUnit {
   int32_t r_lo, r_hi, s, e1;
   int32_t cmin, cwin;  // x window staged in LDS (cwin = 0: gathers from x)
-  int32_t pad0, pad1;
+  int32_t z0, z1;      // the unit's nonzeros (clamped to the partition)
 };
+// Kernel 1 stages a unit's products in LDS when it has at most
+// kBlock * kVecItems nonzeros (each thread loads kVecItems of them).
+constexpr int kVecItems = 8;
 
 // Kernel 3 (CSR5-style tiles): tiles [lt0, lt0 + nt) of one partition, in
 // chunks of 4 (one per wave).  Carry slots 2gu (the piece of the row open at

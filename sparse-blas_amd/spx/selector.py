@@ -15,6 +15,7 @@ GPU against soft targets that favour near-best schedules.
 
     python -m spx.selector collect OUT.npz --seeds 0:1000 --schedules single
     python -m spx.selector merge TRAIN.npz PART1.npz PART2.npz ...
+    python -m spx.selector retime DATA.npz OUT.npz --kernels 1,2,3
     python -m spx.selector train TRAIN.npz --model MODEL.npz
     python -m spx.selector evaluate HELDOUT.npz --model MODEL.npz --train TRAIN.npz
 """
@@ -237,23 +238,36 @@ def autotune(make_plan, x_dev, y_dev, candidates, warmup=2, reps=5):
     return min(times, key=times.get), times
 
 
-def exhaustive(A, x=None, warmup=2, reps=5, only=None):
+def exhaustive(A, x=None, warmup=2, reps=5, only=None, threads=6, batch=16):
     """Time every distinct schedule on A (or those in `only`; the rest stay
     NaN) in ms per execute from HIP events; the copies of a kernel-4 schedule
-    share its time.  Large matrices get fewer repetitions."""
+    share its time.  Large matrices get fewer repetitions.  Plans are built
+    `batch` at a time on `threads` host threads (the kernel-4 layout is built
+    on the host), then timed one after another with the GPU otherwise idle."""
+    from concurrent.futures import ThreadPoolExecutor
+
     x = W.vector(A["n"], 17) if x is None else x
     xd, yd = DeviceBuffer.from_array(x), DeviceBuffer(max(A["m"], 1) * 8)
     if A["nnz"] > 2e7:
         warmup, reps = 1, 3
     t = np.full(NSCHED, np.nan)
-    for s in (distinct() if only is None else only):
+
+    def build(s):
         try:
-            P = Plan(A["rowptr"], A["col"], A["val"], n=A["n"], **schedule(s))
+            return s, Plan(A["rowptr"], A["col"], A["val"], n=A["n"], **schedule(s))
         except Exception as e:  # a schedule that cannot be built stays untimed (NaN)
             print(f"schedule {s} {schedule(s)}: {e}", file=sys.stderr, flush=True)
-            continue
-        t[s] = P.time(xd, yd, warmup=warmup, reps=reps)[0]
-        P.close()
+            return s, None
+
+    # kernel-4 plans first: their host builds take longest
+    order = sorted(distinct() if only is None else only, key=lambda s: (s < 72, s))
+    with ThreadPoolExecutor(max(1, threads)) as pool:
+        for i in range(0, len(order), batch):
+            for s, P in list(pool.map(build, order[i:i + batch])):
+                if P is None:
+                    continue
+                t[s] = P.time(xd, yd, warmup=warmup, reps=reps)[0]
+                P.close()
     for s in range(NSCHED):
         t[s] = t[representative(s)]
     return t
@@ -304,38 +318,126 @@ def _zipf_kmax(z, mean, n):
     return hi
 
 
-def collect(seeds, out, schedules="all", **kw):
+GEN_DEFAULT = (1e4, 6e6, 4e7)
+# the generator bounds of data files that do not record theirs
+GEN_LEGACY = (1e4, 2e6, 8e6)
+
+
+def gen_params(f):
+    """(nmin, nmax, nnz_cap) a data file's matrices were drawn with."""
+    return tuple(float(v) for v in f["gen"]) if "gen" in f.files else GEN_LEGACY
+
+
+def collect(seeds, out, schedules="all", prefetch=3, budget=None, **kw):
     """Features and exhaustive timings of training matrices; saved as npz
-    after every matrix.  schedules: "all" distinct ones, or "single" (one
-    partition and kernel 4)."""
+    after every matrix, with the generator bounds in `gen`.  schedules: "all"
+    distinct ones, or "single" (one partition and kernel 4).  The next
+    `prefetch` matrices are generated on host threads while the GPU times
+    the current one.  With a budget (seconds), no matrix is started after it
+    is spent; the seeds done are in the file."""
+    from concurrent.futures import ThreadPoolExecutor
+
     only = None if schedules == "all" else single_partition()
+    nmin, nmax, cap = kw.get("nmin", GEN_DEFAULT[0]), kw.get("nmax", GEN_DEFAULT[1]), kw.get("nnz_cap", GEN_DEFAULT[2])
     F, T, meta = [], [], []
-    for i, seed in enumerate(seeds):
-        t0 = time.time()
-        A, law = training_matrix(seed, **kw)
-        f = features(A)
-        t = exhaustive(A, only=only)
-        F.append(f)
-        T.append(t)
-        meta.append((seed, A["m"], A["nnz"], law))
-        best = int(np.nanargmin(t))
-        print(f"[{i + 1}/{len(seeds)}] seed {seed} {law} m={A['m']} nnz={A['nnz']} best={best} "
-              f"{schedule(best)} {np.nanmin(t) * 1e3:.1f}us ({time.time() - t0:.1f}s)", flush=True)
-        np.savez(out, features=np.array(F, np.float32), times=np.array(T), seeds=np.array([m[0] for m in meta]),
-                 m=np.array([m[1] for m in meta]), nnz=np.array([m[2] for m in meta]),
-                 law=np.array([m[3] for m in meta]))
+    t_start = time.time()
+    with ThreadPoolExecutor(max(1, prefetch)) as pool:
+        futs = {}
+
+        def ask(j):
+            if j < len(seeds) and j not in futs:
+                futs[j] = pool.submit(training_matrix, seeds[j], nmin, nmax, cap)
+
+        for j in range(prefetch):
+            ask(j)
+        for i, seed in enumerate(seeds):
+            if budget is not None and time.time() - t_start > budget:
+                print(f"budget spent after {i} matrices", flush=True)
+                break
+            t0 = time.time()
+            ask(i)
+            A, law = futs.pop(i).result()
+            ask(i + prefetch)
+            tg = time.time() - t0
+            f = features(A)
+            t = exhaustive(A, only=only)
+            F.append(f)
+            T.append(t)
+            meta.append((seed, A["m"], A["nnz"], law))
+            best = int(np.nanargmin(t))
+            print(f"[{i + 1}/{len(seeds)}] seed {seed} {law} m={A['m']} nnz={A['nnz']} best={best} "
+                  f"{schedule(best)} {np.nanmin(t) * 1e3:.1f}us (wait {tg:.1f}s, {time.time() - t0:.1f}s, "
+                  f"total {time.time() - t_start:.0f}s)", flush=True)
+            np.savez(out, features=np.array(F, np.float32), times=np.array(T), seeds=np.array([m[0] for m in meta]),
+                     m=np.array([m[1] for m in meta]), nnz=np.array([m[2] for m in meta]),
+                     law=np.array([m[3] for m in meta]), gen=np.array([nmin, nmax, cap]))
+            del A
+    return out
+
+
+def retime(path, out, kernels, schedules="all", start=0, budget=None, prefetch=3):
+    """Re-time the schedules of `kernels` (after a change to those kernels) on
+    the matrices of data file `path`, from matrix `start` on, keeping every
+    other timing; written to `out` after every matrix (the matrices not yet
+    re-timed keep their old times, and `retimed` counts the done ones)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with np.load(path, allow_pickle=False) as f:
+        cols = {k: f[k] for k in f.files}
+        gen = gen_params(f)
+    pool_s = distinct() if schedules == "all" else single_partition()
+    only = [s for s in pool_s if 1 + min(s, 95) // 24 in kernels]
+    T = cols["times"].copy()
+    seeds = [int(v) for v in cols["seeds"]]
+    done = int(cols.get("retimed", np.array(0))) if start == 0 and "retimed" in cols else start
+    t_start = time.time()
+    with ThreadPoolExecutor(max(1, prefetch)) as pool:
+        futs = {}
+
+        def ask(j):
+            if j < len(seeds) and j not in futs:
+                futs[j] = pool.submit(training_matrix, seeds[j], *gen)
+
+        for j in range(done, done + prefetch):
+            ask(j)
+        for i in range(done, len(seeds)):
+            if budget is not None and time.time() - t_start > budget:
+                print(f"budget spent at matrix {i}", flush=True)
+                break
+            t0 = time.time()
+            ask(i)
+            A, _ = futs.pop(i).result()
+            ask(i + prefetch)
+            if A["nnz"] != int(cols["nnz"][i]):
+                raise RuntimeError(f"seed {seeds[i]}: regenerated nnz {A['nnz']} != recorded {cols['nnz'][i]}")
+            t = exhaustive(A, only=only)
+            for s in range(NSCHED):
+                if 1 + min(s, 95) // 24 in kernels:
+                    T[i, s] = t[s]
+            best = int(np.nanargmin(T[i]))
+            print(f"[{i + 1}/{len(seeds)}] seed {seeds[i]} nnz={A['nnz']} best={best} {schedule(best)} "
+                  f"({time.time() - t0:.1f}s, total {time.time() - t_start:.0f}s)", flush=True)
+            cols["times"] = T
+            cols["retimed"] = np.array(i + 1)
+            np.savez(out, **cols)
+            del A
     return out
 
 
 def merge(parts, out):
-    """Concatenate collected npz files (same fields) into one."""
+    """Concatenate collected npz files (same fields, same generator bounds)
+    into one."""
     keys = ("features", "times", "seeds", "m", "nnz", "law")
     cols = {k: [] for k in keys}
+    gens = set()
     for path in parts:
         with np.load(path, allow_pickle=False) as f:
             for k in keys:
                 cols[k].append(f[k])
-    np.savez(out, **{k: np.concatenate(v) for k, v in cols.items()})
+            gens.add(gen_params(f))
+    if len(gens) != 1:
+        raise ValueError(f"parts drawn with different generator bounds: {sorted(gens)}")
+    np.savez(out, gen=np.array(gens.pop()), **{k: np.concatenate(v) for k, v in cols.items()})
     return out
 
 
@@ -353,6 +455,7 @@ def main(argv=None):
     c.add_argument("--nmax", type=float, default=6e6)
     c.add_argument("--nnz-cap", type=float, default=4e7)
     c.add_argument("--schedules", choices=["all", "single"], default="all")
+    c.add_argument("--budget", type=float, default=None, help="seconds after which no new matrix is started")
     t = sub.add_parser("train")
     t.add_argument("data", nargs="+")
     t.add_argument("--model", default=MODEL_PATH)
@@ -361,14 +464,24 @@ def main(argv=None):
     e.add_argument("data")
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--train", nargs="*", default=[], help="training sets: the best fixed schedule is chosen on them")
+    r = sub.add_parser("retime")
+    r.add_argument("data")
+    r.add_argument("out")
+    r.add_argument("--kernels", default="1,2,3")
+    r.add_argument("--schedules", choices=["all", "single"], default="all")
+    r.add_argument("--start", type=int, default=0)
+    r.add_argument("--budget", type=float, default=None)
     m = sub.add_parser("merge")
     m.add_argument("out")
     m.add_argument("parts", nargs="+")
     a = ap.parse_args(argv)
     if a.cmd == "merge":
         merge(a.parts, a.out)
+    elif a.cmd == "retime":
+        retime(a.data, a.out, [int(k) for k in a.kernels.split(",")], schedules=a.schedules, start=a.start,
+               budget=a.budget)
     elif a.cmd == "collect":
-        collect(_seeds(a.seeds), a.out, schedules=a.schedules, nmax=a.nmax, nnz_cap=a.nnz_cap)
+        collect(_seeds(a.seeds), a.out, schedules=a.schedules, budget=a.budget, nmax=a.nmax, nnz_cap=a.nnz_cap)
     elif a.cmd == "train":
         X, Tm = [], []
         for path in a.data:

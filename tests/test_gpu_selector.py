@@ -133,11 +133,11 @@ def test_selector_ablation_on_heldout():
     matrices timed on the MI355X.  The features are recomputed on the device
     for the first held-out matrices and must match the recorded ones."""
     with np.load(HELDOUT, allow_pickle=False) as f:
-        X, Tm, seeds = f["features"], f["times"], f["seeds"]
+        X, Tm, seeds, gen = f["features"], f["times"], f["seeds"], S.gen_params(f)
     with np.load(TRAIN, allow_pickle=False) as f:
         Ttr = f["times"]
     for i in range(3):
-        A, _ = S.training_matrix(int(seeds[i]))
+        A, _ = S.training_matrix(int(seeds[i]), *gen)
         assert np.allclose(S.features(A), X[i], rtol=1e-5, atol=1e-6)
     rep = S.report(S.MLP.load(), X, Tm, train_times=Ttr)
     print(rep, flush=True)
