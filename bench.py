@@ -104,6 +104,60 @@ def run_precision(a):
     }), flush=True)
 
 
+def run_c1(a):
+    """The plain method on the n = 8,192 Zipf(1.5) matrix (rows on [1, 512],
+    realized mean about 17.4 nonzeros) and on the reference harness's `g`
+    matrix at n = 10,000: every kernel version with its default schedule,
+    timed warm (the matrices stay in L2 / MALL, so these figures are labelled
+    cache-resident, not HBM), each checked against the oracle, beside the
+    oracle's own time on one host thread."""
+    import spx
+    from spx import workloads as W
+    import oracle
+
+    res = {}
+    for name, A in (("zipf8192", W.zipf(8192, 1.5, 512, seed=1)), ("g10000", W.g_matrix(10_000))):
+        m, n, nnz = A["m"], A["n"], A["nnz"]
+        x = W.vector(n, 3)
+        xd, yd = spx.DeviceBuffer.from_array(x), spx.DeviceBuffer(m * 8)
+        reps = []
+        for i in range(6):
+            t = time.perf_counter()
+            ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+            if i:
+                reps.append(time.perf_counter() - t)
+        cpu_ms = float(np.median(reps)) * 1e3
+        B = W.bytes_model(m, n, nnz, 8)
+        row = {"m": m, "n": n, "nnz": nnz, "mean_row": round(A["mean_row"], 2), "max_row": A["max_row"],
+               "bytes": B, "cpu_1thread_ms": round(cpu_ms, 4), "cpu_1thread_GBs": round(B / (cpu_ms * 1e-3) / 1e9, 2)}
+        for kern in (1, 2, 3, 4):
+            P = spx.Plan(A["rowptr"], A["col"], A["val"], n=n, kernel=kern)
+            ev, kk, wv = P.time_each(xd, yd, 1.0, 0.0, warmup=a.warmup, reps=a.steps)
+            P.execute(xd, yd)
+            spx.synchronize()
+            y = yd.download(np.float64, m)
+            err = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+            if not err <= 1e-13:
+                raise AssertionError(f"{name} kernel {kern}: normwise relative error {err:.3e} > 1e-13")
+            ex = float(np.median(ev))
+            row[f"kernel{kern}"] = {"execute_us_median": round(ex * 1e3, 2), "GBs": round(B / (ex * 1e-3) / 1e9, 1),
+                                    "err_vs_oracle": err}
+            P.close()
+        log(f"{name}: " + json.dumps(row))
+        res[name] = row
+    best = min(res["zipf8192"][f"kernel{k}"]["execute_us_median"] for k in (1, 2, 3, 4))
+    print(json.dumps({
+        "metric": "SpMV GB/s on the n = 8,192 Zipf(1.5) matrix (cache-resident, not an HBM figure)",
+        "value": round(res["zipf8192"]["bytes"] / (best * 1e-6) / 1e9, 1), "unit": "GB/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(best * 1e-3, 6), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "n=8192 Zipf(1.5) rows on [1, 512] (seed 1); reference g matrix n=10000"},
+        "matrices": res,
+        "cpu_baseline": {"value": res["zipf8192"]["cpu_1thread_GBs"], "unit": "GB/s", "cores": 1, "kind": "port",
+                         "sample": "whole matrix, median of 5 reps after 1 warm-up; main.cu:350-356 loop restated in C"},
+    }), flush=True)
+
+
 def run_spmm(a):
     """Multi-RHS SpMM on the circuit-like matrix: the fused passes against
     looping the single-vector execute, k columns, column-major X and Y."""
@@ -309,7 +363,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
-    ap.add_argument("--workload", choices=["circuit", "batch", "rail", "asic", "spmm"], default="circuit",
+    ap.add_argument("--workload", choices=["circuit", "batch", "rail", "asic", "spmm", "c1"], default="circuit",
                     help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain; "
                          "rail / asic: the precision form (FP32 storage) vs plain FP64; spmm: fused multi-RHS")
     ap.add_argument("--k", type=int, default=8, help="spmm: right-hand sides")
@@ -332,6 +386,9 @@ def main():
         return
     if a.workload == "spmm":
         run_spmm(a)
+        return
+    if a.workload == "c1":
+        run_c1(a)
         return
 
     import spx

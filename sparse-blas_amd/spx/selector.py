@@ -375,11 +375,14 @@ def collect(seeds, out, schedules="all", prefetch=3, budget=None, **kw):
     return out
 
 
-def retime(path, out, kernels, schedules="all", start=0, budget=None, prefetch=3):
+def retime(path, out, kernels, schedules="all", start=0, budget=None, prefetch=3, min_n=0, resume=False):
     """Re-time the schedules of `kernels` (after a change to those kernels) on
     the matrices of data file `path`, from matrix `start` on, keeping every
     other timing; written to `out` after every matrix (the matrices not yet
-    re-timed keep their old times, and `retimed` counts the done ones)."""
+    re-timed keep their old times, and `retimed` counts the done ones;
+    `resume` continues from there).  Matrices with fewer than `min_n` rows
+    (square: columns) are skipped, for a change that only affects larger
+    ones."""
     from concurrent.futures import ThreadPoolExecutor
 
     with np.load(path, allow_pickle=False) as f:
@@ -389,25 +392,28 @@ def retime(path, out, kernels, schedules="all", start=0, budget=None, prefetch=3
     only = [s for s in pool_s if 1 + min(s, 95) // 24 in kernels]
     T = cols["times"].copy()
     seeds = [int(v) for v in cols["seeds"]]
-    done = int(cols.get("retimed", np.array(0))) if start == 0 and "retimed" in cols else start
+    todo = [i for i in range(len(seeds)) if int(cols["m"][i]) >= min_n]
+    done = int(cols["retimed"]) if resume and "retimed" in cols else start
     t_start = time.time()
     with ThreadPoolExecutor(max(1, prefetch)) as pool:
         futs = {}
 
-        def ask(j):
-            if j < len(seeds) and j not in futs:
-                futs[j] = pool.submit(training_matrix, seeds[j], *gen)
+        order = [i for i in todo if i >= done]
 
-        for j in range(done, done + prefetch):
-            ask(j)
-        for i in range(done, len(seeds)):
+        def ask(q):
+            if q < len(order) and order[q] not in futs:
+                futs[order[q]] = pool.submit(training_matrix, seeds[order[q]], *gen)
+
+        for q in range(prefetch):
+            ask(q)
+        for q, i in enumerate(order):
             if budget is not None and time.time() - t_start > budget:
                 print(f"budget spent at matrix {i}", flush=True)
                 break
             t0 = time.time()
-            ask(i)
+            ask(q)
             A, _ = futs.pop(i).result()
-            ask(i + prefetch)
+            ask(q + prefetch)
             if A["nnz"] != int(cols["nnz"][i]):
                 raise RuntimeError(f"seed {seeds[i]}: regenerated nnz {A['nnz']} != recorded {cols['nnz'][i]}")
             t = exhaustive(A, only=only)
@@ -421,6 +427,9 @@ def retime(path, out, kernels, schedules="all", start=0, budget=None, prefetch=3
             cols["retimed"] = np.array(i + 1)
             np.savez(out, **cols)
             del A
+        else:
+            cols["retimed"] = np.array(len(seeds))
+            np.savez(out, **cols)
     return out
 
 
@@ -471,6 +480,8 @@ def main(argv=None):
     r.add_argument("--schedules", choices=["all", "single"], default="all")
     r.add_argument("--start", type=int, default=0)
     r.add_argument("--budget", type=float, default=None)
+    r.add_argument("--min-n", type=int, default=0, help="skip matrices with fewer rows")
+    r.add_argument("--resume", action="store_true", help="continue an interrupted re-timing of the same file")
     m = sub.add_parser("merge")
     m.add_argument("out")
     m.add_argument("parts", nargs="+")
@@ -479,7 +490,7 @@ def main(argv=None):
         merge(a.parts, a.out)
     elif a.cmd == "retime":
         retime(a.data, a.out, [int(k) for k in a.kernels.split(",")], schedules=a.schedules, start=a.start,
-               budget=a.budget)
+               budget=a.budget, min_n=a.min_n, resume=a.resume)
     elif a.cmd == "collect":
         collect(_seeds(a.seeds), a.out, schedules=a.schedules, budget=a.budget, nmax=a.nmax, nnz_cap=a.nnz_cap)
     elif a.cmd == "train":
