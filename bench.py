@@ -212,8 +212,9 @@ def run_batch(a):
     against the learned-selector + FP32-storage batched form (all matrices
     in one plan, one hipGraph replay per batch; the schedule is the fastest of
     the selector's shortlist timed on a sample of the batch).
-    Prints one JSON line: matrices/s of both, the ratio, and the FP32 form's
-    normwise error against the oracle on sampled members."""
+    Prints one JSON line: matrices/s of both, the ratio, the same against
+    looping the drop-in itself (host arrays per call, on every 8th member),
+    and the FP32 form's normwise error against the oracle on sampled members."""
     import ctypes as C
 
     import spx
@@ -253,6 +254,19 @@ def run_batch(a):
         P.close()
     del plans
     log(f"plain: {plain_ms:.3f} ms per batch")
+    # the literal baseline: looping the plain drop-in (spMV_mgpu_v1, kernel 2,
+    # host arrays in and out per call) over every 8th member, as matrices/s
+    t = time.perf_counter()
+    nd = 0
+    for A in mats[::8]:
+        yh = np.zeros(A["m"])
+        rc, _ = spx.spMV_mgpu_v1(A["m"], A["n"], A["nnz"], 1.0, A["val"], A["rowptr"], A["col"], x[:A["n"]], 0.0,
+                                 yh, 1, kernel=2)
+        if rc != 0:
+            raise RuntimeError(f"spMV_mgpu_v1 returned {rc}")
+        nd += 1
+    dropin_mps = nd / (time.perf_counter() - t)
+    log(f"drop-in loop: {dropin_mps:.1f} matrices/s over {nd} members")
 
     # learned selector on the packed block-diagonal matrix
     t0 = time.time()
@@ -325,6 +339,8 @@ def run_batch(a):
         "autotune": {"shortlist": [int(c) for c in cands], "chosen": int(sched),
                      "sample_ms": {str(k): round(v, 4) for k, v in ttimes.items()}},
         "plain_fp64_matrices_per_s": round(len(mats) / (plain_ms * 1e-3), 1),
+        "dropin_loop_matrices_per_s": round(dropin_mps, 1),
+        "speedup_vs_dropin_loop": round(len(mats) / (ms32 * 1e-3) / dropin_mps, 1),
         "batched_fp64_matrices_per_s": round(len(mats) / (ms64 * 1e-3), 1),
         "speedup_vs_plain": round(plain_ms / ms32, 3),
         "fp32_GBs": round(B32 / (ms32 * 1e-3) / 1e9, 1), "fp64_batched_GBs": round(B64 / (ms64 * 1e-3) / 1e9, 1),

@@ -560,6 +560,9 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
   __shared__ int s_slot;
   double *const s_p = s_x;
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+  // the start bits are cleared here and again after each unit has read them
+  for (int w = tid; w < TW / 32 + 2; w += kBlock) s_fl[w] = 0u;
+  __syncthreads();
   int64_t iter = 0;
   int64_t u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
   int dv = desc_prefetch(a.munits + a.unit_begin, u, a.unit_count, lane);
@@ -573,11 +576,9 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
     const int nrows = r1 - i0, nnzs = j1 - j0;
     const bool stg = cwin > 0;
     int lead = 0;
-    // the x window, the row ends and the unit's nonzeros are all requested
+    // the x window, the unit's nonzeros and its row ends are all requested
     // before the first wait
     if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
-    for (int k = tid; k <= nrows; k += kBlock) s_re[k] = a.rowptr[i0 + k];
-    for (int w = tid; w < TW / 32 + 2; w += kBlock) s_fl[w] = 0u;
     int ci[IPT];
     V vv[IPT];
     __builtin_amdgcn_s_setprio(2);
@@ -588,22 +589,24 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
       vv[r] = k < nnzs ? __builtin_nontemporal_load(a.val + j0 + k) : (V)0;
     }
     __builtin_amdgcn_s_setprio(0);
-    if (stg) stage_wait();
-    __syncthreads();  // the window, the row ends and the cleared start bits are in LDS
     // a row end at unit position p < nnzs: nonzero p starts a row
-    for (int k = tid; k < nrows; k += kBlock) {
-      const int p = s_re[k + 1] - j0;
-      if (p >= 0 && p < nnzs) atomicOr(&s_fl[p >> 5], 1u << (p & 31));
+    for (int k = tid; k <= nrows; k += kBlock) {
+      const int re = a.rowptr[i0 + k];
+      s_re[k] = re;
+      const int p = re - j0;
+      if (k > 0 && p >= 0 && p < nnzs) atomicOr(&s_fl[p >> 5], 1u << (p & 31));
     }
     double pv[IPT];
     if (stg) {
+      stage_wait();
+      __syncthreads();  // the window is in LDS
 #pragma unroll
       for (int r = 0; r < IPT; r++) pv[r] = prod(vv[r], s_x[lead + (tid + r * kBlock < nnzs ? ci[r] - cmin : 0)]);
+      __syncthreads();  // every gather is done: the products overwrite the window
     } else {
 #pragma unroll
       for (int r = 0; r < IPT; r++) pv[r] = prod(vv[r], a.x[ci[r]]);
     }
-    __syncthreads();  // every gather is done: the products overwrite the window
 #pragma unroll
     for (int r = 0; r < IPT; r++) {
       const int k = tid + r * kBlock;
@@ -636,7 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_merge(KArgs<V> a) {
       s_wv[wid] = v;
       s_ws[wid] = sg;
     }
-    __syncthreads();
+    __syncthreads();  // also: every thread has read its start bits
+    for (int w = tid; w < TW / 32 + 2; w += kBlock) s_fl[w] = 0u;
     double prev = 0.0;
 #pragma unroll
     for (int w = 0; w < NW; w++)

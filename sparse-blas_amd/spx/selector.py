@@ -525,7 +525,9 @@ def report(net, X, Tm, train_times=None, nnz=None):
     selector's mean regret per nnz decade."""
     ch = net.choose(X)
     ref = Tm if train_times is None else train_times
-    fixed = int(np.nanargmin(np.nanmean(ref / np.nanmin(ref, axis=1, keepdims=True), axis=0)))
+    rel = ref / np.nanmin(ref, axis=1, keepdims=True)
+    timed = ~np.isnan(rel).all(axis=0)  # schedules the reference set timed
+    fixed = int(np.flatnonzero(timed)[np.argmin(np.nanmean(rel[:, timed], axis=0))])
     r = regret(Tm, ch)
     best = np.nanargmin(Tm, axis=1)
     multi = np.array([s < 72 and s % 6 != 0 for s in best])
