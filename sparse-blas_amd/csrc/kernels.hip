@@ -377,13 +377,19 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
   __shared__ __attribute__((aligned(16))) double s_x[XS];
   __shared__ int s_slot;
   const int tid = threadIdx.x, g = tid / L, l = tid % L, lane = tid % kWave;
+  // rows differ in length, so without the queue the XCDs take unit ranges
+  // of about equal nonzeros (vxr) rather than equal unit counts
   int64_t iter = 0;
-  int64_t u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+  auto next = [&]() {
+    return a.vxr && !a.queue ? next_unit_ranged(a.vxr, iter++, a.unit_count)
+                             : next_unit(a.queue, iter++, a.unit_count, &s_slot);
+  };
+  int64_t u = next();
   int dv = desc_prefetch(a.vunits + a.unit_begin, u, a.unit_count, lane);
   while (u < a.unit_count) {
     const int dcur = dv;
     const int64_t gu = a.unit_begin + u;
-    u = next_unit(a.queue, iter++, a.unit_count, &s_slot);
+    u = next();
     dv = desc_prefetch(a.vunits + a.unit_begin, u, a.unit_count, lane);
     const int r_lo = desc_field(dcur, 0), r_hi = desc_field(dcur, 1), s0 = desc_field(dcur, 2),
               e1 = desc_field(dcur, 3), cmin = desc_field(dcur, 4), cwin = desc_field(dcur, 5),
@@ -477,16 +483,29 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
           const double t1 = prod(v1[k], gx(c[k].y, ok1));
           sum[k] = (ok0 ? t0 : 0.0) + (ok1 ? t1 : 0.0);
         }
-        // rows longer than the first step
+        // rows longer than the first step: four steps' loads at a time (a
+        // step past the row's end reloads the block's first pair), added in
+        // step order
 #pragma unroll
         for (int k = 0; k < R; k++) {
-          for (int p = (b[k] >> 1) + l + L; 2 * p < e[k]; p += L) {
-            V v0, v1;
-            Pair<V>::load(a.val, 2 * (int64_t)p, v0, v1);
-            const vi2 c = *reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)p);
-            const bool ok1 = 2 * p + 1 < e[k];
-            const double t0 = prod(v0, gx(c.x, true)), t1 = prod(v1, gx(c.y, ok1));
-            sum[k] += t0 + (ok1 ? t1 : 0.0);
+          for (int p = (b[k] >> 1) + l + L; 2 * p < e[k]; p += 4 * L) {
+            V v0[4], v1[4];
+            vi2 c[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int q = 2 * (p + u * L) < e[k] ? p + u * L : p;
+              Pair<V>::load(a.val, 2 * (int64_t)q, v0[u], v1[u]);
+              c[u] = __builtin_nontemporal_load(reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)q));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int q = p + u * L;
+              if (2 * q < e[k]) {
+                const bool ok1 = 2 * q + 1 < e[k];
+                const double t0 = prod(v0[u], gx(c[u].x, true)), t1 = prod(v1[u], gx(c[u].y, ok1));
+                sum[k] += t0 + (ok1 ? t1 : 0.0);
+              }
+            }
           }
         }
       };

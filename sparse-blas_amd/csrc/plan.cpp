@@ -91,6 +91,8 @@ struct spx_plan {
   double *d_val = nullptr;       // FP64 values (CSR or tile layout)
   float *d_val32 = nullptr;      // FP32 values (precision form)
   VUnit *d_vunits = nullptr;     // kernel 1: unit table
+  int32_t *d_vxr = nullptr;      // kernel 1: per launch, kXcd + 1 unit bounds balanced by nonzeros
+  std::vector<int64_t> vnnz;     // kernel 1: nonzeros per unit (plan time only)
   MUnit *d_munits = nullptr;     // kernel 2: unit table
   Part *d_parts = nullptr;
   std::vector<Part> parts;
@@ -155,7 +157,7 @@ struct spx_plan {
   double upload_ms = 0.0, build_ms = 0.0;  // host time: copies of the input, the whole build
 
   ~spx_plan() {
-    for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_vunits, (void *)d_munits,
+    for (void *p : {(void *)d_rowptr, (void *)d_col, (void *)d_val, (void *)d_val32, (void *)d_vunits, (void *)d_vxr, (void *)d_munits,
                     (void *)d_parts, (void *)d_units, (void *)d_coords, (void *)d_tp, (void *)d_tf,
                     (void *)d_crow, (void *)d_cval, (void *)d_tunits, (void *)d_empty, (void *)d_queue,
                     (void *)d_lrows, (void *)d_partial, (void *)d_sunits, (void *)d_sxr, (void *)d_sstart, (void *)d_partk, (void *)d_lsp, (void *)d_lseg,
@@ -1258,6 +1260,8 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
         v.cmin = v.z0 = std::max<int32_t>(rp[v.r_lo], v.s);
         v.cwin = v.z1 = std::min<int32_t>(rp[v.r_hi], v.e1);
       }
+    P->vnnz.resize((size_t)P->nunits);
+    for (int64_t u = 0; u < P->nunits; u++) P->vnnz[(size_t)u] = (int64_t)vu[(size_t)u].z1 - vu[(size_t)u].z0;
     if ((rc = P->upload(&P->d_vunits, vu.data(), P->nunits))) return rc;
     HIP_TRY(launch_unit_window(P->d_col, reinterpret_cast<int32_t *>(P->d_vunits), 8, 4, P->nunits, 0));
   }
@@ -1366,6 +1370,22 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
       if (q.units == 0) continue;
       P->launches.push_back({(int)(pi % nstreams), q.unit_off, q.units, direct_grid(q.units)});
     }
+    if (P->kernel == SPX_KERNEL_VECTOR && !P->launches.empty()) {
+      // each launch's units split over the XCDs by nonzeros (plus a fixed
+      // cost per unit): contiguous ranges, so neighbouring rows share an L2
+      std::vector<int32_t> xr;
+      for (auto &L : P->launches) {
+        std::vector<double> w((size_t)L.ucnt + 1, 0.0);
+        for (int64_t u = 0; u < L.ucnt; u++) w[(size_t)u + 1] = w[(size_t)u] + (double)P->vnnz[(size_t)(L.ubeg + u)] + 64.0;
+        xr.push_back(0);
+        for (int x = 1; x < kXcd; x++)
+          xr.push_back(std::max<int32_t>(xr.back(), (int32_t)(std::lower_bound(w.begin(), w.end() - 1, w[(size_t)L.ucnt] * x / kXcd) - w.begin())));
+        xr.push_back((int32_t)L.ucnt);
+      }
+      if ((rc = P->upload(&P->d_vxr, xr.data(), (int64_t)xr.size()))) return rc;
+    }
+    P->vnnz.clear();
+    P->vnnz.shrink_to_fit();
   }
   if (nstreams > 1) {
     P->streams.resize(nstreams);
@@ -1408,6 +1428,7 @@ static hipError_t launch_main(spx_plan *P, const KArgs<V> &base, const spx_plan:
   a.unit_begin = L.ubeg;
   a.unit_count = L.ucnt;
   a.queue = nullptr;
+  if (P->d_vxr) a.vxr = P->d_vxr + (&L - P->launches.data()) * (kXcd + 1);
   if (P->mode == SPX_MODE_QUEUE) {
     a.queue = P->d_queue + (int64_t)L.stream * kXcd;
     hipError_t e = launch_reset(a.queue, kXcd, st);

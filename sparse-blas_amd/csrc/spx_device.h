@@ -125,6 +125,15 @@ __device__ inline int64_t next_unit(int *queue, int64_t iter, int64_t count, int
   return u;
 }
 
+// Next unit for this workgroup from a per-XCD range table (the host split
+// the units into kXcd contiguous ranges of about equal work): workgroup b
+// walks XCD (b mod kXcd)'s range round-robin.  Returns count when done.
+__device__ inline int64_t next_unit_ranged(const int32_t *xr, int64_t iter, int64_t count) {
+  const int x = blockIdx.x % kXcd;
+  const int64_t u = xr[x] + (int64_t)(blockIdx.x / kXcd) + iter * (int64_t)(gridDim.x / kXcd);
+  return u < xr[x + 1] ? u : count;
+}
+
 // The segmented engine's units for this workgroup.  XCD x owns units
 // [xr[x], xr[x+1]), ranges the host balanced by work, and its
 // workgroups walk them round-robin in order, so neighbouring units (which

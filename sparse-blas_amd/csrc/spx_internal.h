@@ -213,6 +213,7 @@ struct KArgs {
   int32_t *crow;
   double *cval;
   const VUnit *vunits;       // kernel 1
+  const int32_t *vxr;        // kernel 1 without the queue: XCD x walks units [vxr[x], vxr[x+1])
   const MUnit *munits;       // kernel 2
   const TUnit *tunits;       // kernel 3
   const uint32_t *tile_ptr;  // kernel 3
