@@ -1211,7 +1211,14 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
   // units never cross partitions; every unit u (global numbering, in
   // nonzero order) owns carry slots 2u and 2u + 1
   const int tw = kBlock * P->items;
-  const int64_t tiles_per_unit = 4 * kTileUnitChunks;
+  // kernel 3: up to kTileUnitChunks chunks of 4 tiles per unit, fewer when the
+  // matrix has too few tiles to give about 1,024 units
+  int64_t tiles_per_unit = 4 * kTileUnitChunks;
+  if (P->kernel == SPX_KERNEL_TILE) {
+    int64_t tiles = 0;
+    for (const auto &q : P->parts) tiles += (q.e - q.s + 1 + (int64_t)kOmega * P->sigma - 1) / ((int64_t)kOmega * P->sigma);
+    tiles_per_unit = 4 * std::min<int64_t>(kTileUnitChunks, std::max<int64_t>(1, tiles / (4 * 1024)));
+  }
   int64_t uoff = 0, cooff = 0, toff = 0, tpoff = 0;
   for (auto &q : P->parts) {
     const int64_t rows = (int64_t)q.row_hi - q.row_lo;
