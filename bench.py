@@ -49,14 +49,15 @@ def profiled_traffic(kernel, items, window):
 
 
 def run_precision(a):
-    """The precision form on the rail-like (4,284 x 1,092,610, values 1.0) or
-    ASIC-like (682,862 square, lognormal values with random sign) matrix:
-    plain FP64 and FP32-storage plans of the same schedule timed back to back,
-    each with its normwise and worst-row relative error against the oracle."""
+    """The precision form on the rail-like (4,284 x 1,092,610, values 1.0),
+    ASIC-like (682,862 square, lognormal values with random sign) or circuit-like
+    (5,558,326 square, N(0,1) values; `prec-circuit`) matrix: plain FP64 and
+    FP32-storage plans of the same schedule timed back to back, each with its
+    normwise and worst-row relative error against the oracle."""
     import spx
     from spx import workloads as W
 
-    A = W.rail() if a.workload == "rail" else W.asic()
+    A = {"rail": W.rail, "asic": W.asic, "prec-circuit": W.circuit}[a.workload]()
     m, n, nnz = A["m"], A["n"], A["nnz"]
     log(f"workload {a.workload}: m={m} n={n} nnz={nnz} mean_row={A['mean_row']:.2f} max_row={A['max_row']}")
     x = W.vector(n, 31)
@@ -379,7 +380,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the "
                          "known-size copy that scales the FETCH_SIZE/WRITE_SIZE counters")
-    ap.add_argument("--workload", choices=["circuit", "batch", "rail", "asic", "spmm", "c1"], default="circuit",
+    ap.add_argument("--workload", choices=["circuit", "batch", "rail", "asic", "prec-circuit", "spmm", "c1"], default="circuit",
                     help="circuit: the 5.5M-row SpMV (default); batch: 512 matrices, batched form vs plain; "
                          "rail / asic: the precision form (FP32 storage) vs plain FP64; spmm: fused multi-RHS")
     ap.add_argument("--k", type=int, default=8, help="spmm: right-hand sides")
@@ -397,7 +398,7 @@ def main():
     if a.workload == "batch":
         run_batch(a)
         return
-    if a.workload in ("rail", "asic"):
+    if a.workload in ("rail", "asic", "prec-circuit"):
         run_precision(a)
         return
     if a.workload == "spmm":

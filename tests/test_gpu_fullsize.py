@@ -5,6 +5,8 @@
   spMV_mgpu_v2 with the harness's (d, c) arguments, csr5_kernel and its
   no-alloc twin) and the plan API's kernel 1, against the FP64 oracle
   (main.cu:343-357; SB/test/dspmv_test.cu:355-382 drives the same calls).
+- The same generator with uniform columns (kernel 4's column bands, and the
+  merge path).
 - The batched form on 512 uneven matrices (about 1.2e9 nonzeros in one
   plan, replayed as a hipGraph), sampled members against the oracle.
 - Fused multi-RHS SpMM with k = 8 and k = 32 on the circuit matrix, sampled
@@ -45,6 +47,22 @@ def check_circuit(A, x, y, ref, alpha=1.0):
         mag = abs(alpha) * np.abs(A["val"][lo:hi] * x[A["col"][lo:hi]]).sum()
         assert abs(y[r] - ref[r]) <= 2 * k[r] * U / (1 - k[r] * U) * mag
     return rel
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("kernel", [4, 2])
+def test_c2_uniform_columns(kernel):
+    """The same generator with every regular row's columns uniform over
+    [0, n): kernel 4 cuts those rows at column-band edges (22 bands), the
+    merge path gathers from all of x."""
+    A = W.circuit(window=0)
+    x = W.vector(A["n"], 24)
+    ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
+    print(f"c2 uniform columns, kernel {kernel}", flush=True)
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=kernel)
+    if kernel == 4:
+        assert P.info()["bands"] == 22
+    check_circuit(A, x, P(x, 1.0, 0.0), ref)
 
 
 def test_c2_plan_kernel1(circuit):
