@@ -9,8 +9,8 @@
  * (0-based, general).  Pointers are HOST memory except in csr5_kernel.
  * Return codes follow the reference: 0 success; -1 a bad argument (ngpu != 1,
  * kernel not in {1,2,3}, nb <= 0, copy_of_workspace <= 0), a footprint above
- * 0.8 of free device memory, or a failed computation; 1 a device allocation
- * or copy failure.  Each entry point is a thin wrapper over the extern "C"
+ * 0.8 of free device memory (baseline and v1; v2 then streams its tasks), or
+ * a failed computation; 1 a device allocation or copy failure.  Each entry point is a thin wrapper over the extern "C"
  * ABI in spx_capi.h.
  *
  * Differences from the reference (all deliberate):
@@ -51,7 +51,10 @@ int spMV_mgpu_v1(int m, int n, long long nnz, double *alpha,
 
 /* SparseBLAS/include/spmv_kernel.h:21-28: task pool of nb-nonzero tasks over
  * copy_of_workspace streams; here a device work queue drained by persistent
- * workgroups. */
+ * workgroups.  nb is capped at 0.8 of free memory / 16 bytes / copies
+ * (SparseBLAS/src/dspmv_mgpu_v2.cu:43); a matrix whose footprint exceeds
+ * 0.8 of free memory runs its tasks one after another, at most nb nonzeros
+ * on the device at a time (v2:108-115). */
 int spMV_mgpu_v2(int m, int n, long long nnz, double *alpha,
                  double *csrVal, long long *csrRowPtr, int *csrColIndex,
                  double *x, double *beta,

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <limits>
+#include <vector>
 
 #include "../../include/spmv_kernel.h"
 #include "../../include/spx_capi.h"
@@ -88,6 +89,83 @@ int run_host(const spx_plan_desc &d, double alpha, const double *x, double beta,
   if (dy) spx_free(dy);
   spx_plan_destroy(plan);
   g_times[5] = ms_since(t);
+  g_times[6] = ms_since(t_all);
+  return ret;
+}
+
+// spMV_mgpu_v2 for a matrix whose footprint exceeds the memory guard: its
+// tasks (SB/src/dspmv_mgpu_v2.cu:218-267: T = ceil(nnz / nb) nearly equal
+// nonzero ranges, start = floor(t nnz / T)) run one after another through
+// plans that each hold one task's nonzeros, so no more than nb of them are
+// on the device at once (v2:108-115 allocates nb-sized buffers per stream
+// and streams the tasks through them).  x is uploaded once; each task's row
+// sums (alpha applied) come back and are added on the host in task order,
+// and beta * y is added last (v2:381-437 gathers the tasks' results on the
+// host likewise).
+int run_streamed(int m, int n, long long nnz, double alpha, const double *val, const long long *rowptr,
+                 const int *col, const double *x, double beta, double *y, int kernel, long long nb) {
+  const auto t_all = std::chrono::steady_clock::now();
+  for (double &t : g_times) t = 0.0;
+  const long long T = (nnz + nb - 1) / nb;
+  std::vector<double> acc((size_t)m, 0.0);
+  double *dx = (double *)spx_malloc((int64_t)std::max(n, 1) * 8);
+  double *dy = nullptr;
+  int ret = 0;
+  if (!dx || (n > 0 && spx::stage_h2d(dx, x, (size_t)n * 8) != hipSuccess)) ret = 1;
+  std::vector<long long> lrp;
+  std::vector<double> part;
+  int64_t dy_rows = 0;
+  for (long long t = 0; t < T && ret == 0; t++) {
+    const long long s0 = (long long)std::floor((double)(t * nnz / T));
+    const long long e1 = (long long)std::floor((double)((t + 1) * nnz / T));  // one past the task's last nonzero
+    if (e1 <= s0) continue;
+    // rows holding the task's first and last nonzeros: upper_bound - 1
+    const int r0 = (int)(std::upper_bound(rowptr, rowptr + m + 1, s0) - rowptr) - 1;
+    const int r1 = (int)(std::upper_bound(rowptr, rowptr + m + 1, e1 - 1) - rowptr) - 1;
+    const int mt = r1 - r0 + 1;
+    lrp.assign((size_t)mt + 1, 0);
+    for (int i = 0; i <= mt; i++)
+      lrp[(size_t)i] = std::min<long long>(std::max<long long>(rowptr[r0 + i] - s0, 0), e1 - s0);
+    if (mt > dy_rows) {
+      if (dy) spx_free(dy);
+      dy_rows = mt;
+      dy = (double *)spx_malloc(dy_rows * 8);
+      if (!dy) {
+        ret = 1;
+        break;
+      }
+    }
+    spx_plan_desc d;
+    spx_plan_desc_init(&d);
+    d.m = mt; d.n = n; d.nnz = e1 - s0;
+    d.rowptr64 = reinterpret_cast<const int64_t *>(lrp.data());
+    d.col = col + s0; d.val = val + s0;
+    d.kernel = kernel;
+    d.mode = SPX_MODE_NNZ;
+    d.parts = 1;
+    spx_plan_t plan = nullptr;
+    int rc = spx_plan_create(&plan, &d);
+    if (rc == SPX_OK) {
+      rc = spx_plan_execute(plan, alpha, dx, 0.0, dy, nullptr);
+      if (rc == SPX_OK) rc = spx_synchronize();
+      spx_plan_destroy(plan);
+    }
+    if (rc != SPX_OK) {
+      std::fprintf(stderr, "spx: streamed task %lld failed: %s\n", t, spx_last_error());
+      ret = code_of(rc);
+      break;
+    }
+    part.resize((size_t)mt);
+    if (spx::stage_d2h(part.data(), dy, (size_t)mt * 8) != hipSuccess) {
+      ret = 1;
+      break;
+    }
+    for (int i = 0; i < mt; i++) acc[(size_t)(r0 + i)] += part[(size_t)i];
+  }
+  if (dx) spx_free(dx);
+  if (dy) spx_free(dy);
+  if (ret == 0)
+    for (int r = 0; r < m; r++) y[r] = beta == 0.0 ? acc[(size_t)r] : acc[(size_t)r] + beta * y[r];
   g_times[6] = ms_since(t_all);
   return ret;
 }
@@ -209,6 +287,9 @@ int spx_dropin_v2(int m, int n, long long nnz, double *alpha, double *csrVal, lo
                               (double)(sizeof(double) + sizeof(int) + sizeof(int))) / q;
   nb = std::min(nb, cap);
   if (nb <= 0) return -1;
+  // a matrix beyond the guard streams through nb-sized tasks (v2:108-115)
+  if (footprint_gb(nnz, m, n) > 0.8 * spx_get_gpu_availble_mem(ngpu))
+    return run_streamed(m, n, nnz, *alpha, csrVal, csrRowPtr, csrColIndex, x, *beta, y, kernel, nb);
   spx_plan_desc d;
   spx_plan_desc_init(&d);
   d.m = m; d.n = n; d.nnz = nnz;

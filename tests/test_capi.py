@@ -123,6 +123,12 @@ def test_dropin_returns_1_when_a_device_allocation_fails():
             L.spx_debug_fail_alloc(0)
             assert call()[0] == 1
             assert b"out of memory" in L.spx_last_error().lower() or b"memory" in L.spx_last_error().lower()
+        # v2 beyond the memory guard streams its tasks: the streamed path's
+        # first allocation fails the same way
+        L.spx_debug_set_free_mem(50)  # 0.8 * 50 B < the 80 B footprint; nb capped at 2
+        L.spx_debug_fail_alloc(0)
+        assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=2, nb=100, q=1)[0] == 1
+        L.spx_debug_set_free_mem(1 << 40)
         # a plan fails with SPX_EMEM the same way, and the seam disarms after firing
         L.spx_debug_fail_alloc(0)
         h = C.c_void_p()

@@ -258,6 +258,28 @@ def test_dropin_nonzero_beta_and_v1_streams():
         spx.set_v1_streams(old)
 
 
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+def test_dropin_v2_streams_tasks_beyond_the_memory_guard(kernel):
+    """A matrix whose footprint exceeds 0.8 x free memory: spMV_mgpu_v2
+    runs its nb-sized tasks one after another (SB/src/dspmv_mgpu_v2.cu:43,
+    108-115) instead of refusing, and the result matches the oracle; v1 and
+    the baseline still refuse (-1).  Free memory is set through the seam."""
+    A = W.zipf(20000, 1.5, 512, seed=13)
+    x = W.vector(A["n"], 2)
+    y0 = W.vector(A["m"], 3)
+    L = spx.lib()
+    try:
+        L.spx_debug_set_free_mem(2_000_000)  # the footprint is about 4.6 MB
+        args = (A["m"], A["n"], A["nnz"], 0.7, A["val"], A["rowptr"], A["col"], x, -0.3, y0)
+        assert spx.spMV_mgpu_v1(*args, kernel=kernel)[0] == -1
+        for q in (1, 2):  # nb is capped at 0.8 * 2e6 / 16 / q: about 14 and 28 tasks
+            rc, y = spx.spMV_mgpu_v2(*args, ngpu=1, kernel=kernel, nb=A["nnz"], q=q)
+            assert rc == 0
+            check_fp64(A, x, y, 0.7, -0.3, y0)
+    finally:
+        L.spx_debug_set_free_mem(-1)
+
+
 def test_csr5_kernel_device_pointers():
     A = W.zipf(4000, 1.5, 512, seed=12)
     x = W.vector(A["n"], 4)
