@@ -106,6 +106,12 @@ int run_streamed(int m, int n, long long nnz, double alpha, const double *val, c
                  const int *col, const double *x, double beta, double *y, int kernel, long long nb) {
   const auto t_all = std::chrono::steady_clock::now();
   for (double &t : g_times) t = 0.0;
+  // the tasks' row ranges come from the row pointer: it must run
+  // monotonically from 0 to nnz
+  if (m > 0 && (!rowptr || rowptr[0] != 0 || rowptr[m] != nnz)) return -1;
+  for (int r = 0; r < m; r++)
+    if (rowptr[r + 1] < rowptr[r]) return -1;
+  if (nnz > 0 && (!val || !col)) return -1;
   const long long T = (nnz + nb - 1) / nb;
   std::vector<double> acc((size_t)m, 0.0);
   double *dx = (double *)spx_malloc((int64_t)std::max(n, 1) * 8);

@@ -128,6 +128,10 @@ def test_dropin_returns_1_when_a_device_allocation_fails():
         L.spx_debug_set_free_mem(50)  # 0.8 * 50 B < the 80 B footprint; nb capped at 2
         L.spx_debug_fail_alloc(0)
         assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, rp, c, x, 0.0, y, kernel=2, nb=100, q=1)[0] == 1
+        # a row pointer that does not run monotonically from 0 to nnz is refused first
+        L.spx_debug_fail_alloc(-1)
+        bad = np.array([0, 3, 2], np.int64)
+        assert spx.spMV_mgpu_v2(m, n, nnz, 1.0, v, bad, c, x, 0.0, y, kernel=2, nb=100, q=1)[0] == -1
         L.spx_debug_set_free_mem(1 << 40)
         # a plan fails with SPX_EMEM the same way, and the seam disarms after firing
         L.spx_debug_fail_alloc(0)
