@@ -189,7 +189,7 @@ struct spx_plan {
   int upload(T **p, const T *src, int64_t count) {
     int rc = alloc(p, count);
     if (rc) return rc;
-    if (count > 0) HIP_TRY(hipMemcpy(*p, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice));
+    if (count > 0) HIP_TRY(stage_h2d(*p, src, sizeof(T) * (size_t)count));
     return SPX_OK;
   }
 };
@@ -558,10 +558,44 @@ struct InputOnDevice {
   DevTmp<double> val;
 };
 
+// Kernels 1-3: the long rows' ranges in the input and their own row pointer.
+struct LongSplit {
+  std::vector<LongRange> lr;  // input range and destination of each long row, row order
+  std::vector<int32_t> lrp;   // row pointer of the long rows' arrays
+  int64_t ln = 0;             // long-row nonzeros
+};
+
+// Fills S from rp and the long rows (ascending), then turns rp in place
+// into the row pointer of the regular part, where the long rows are empty.
+static void split_long_rows(std::vector<int32_t> &rp, const std::vector<int32_t> &long_rows, LongSplit &S) {
+  const int m = (int)rp.size() - 1, nl = (int)long_rows.size();
+  S.lr.resize((size_t)nl);
+  S.lrp.assign((size_t)nl + 1, 0);
+  S.ln = 0;
+  for (int j = 0; j < nl; j++) {
+    const int r = long_rows[(size_t)j];
+    S.lr[(size_t)j] = LongRange{rp[r], rp[r + 1], S.ln};
+    S.ln += rp[r + 1] - rp[r];
+    S.lrp[(size_t)j + 1] = (int32_t)S.ln;
+  }
+  if (nl == 0) return;
+  size_t j = 0;
+  int32_t sub = 0, prev = rp[0];  // prev: the input's rp[i]
+  for (int i = 0; i < m; i++) {
+    const int32_t next = rp[i + 1];
+    if (j < long_rows.size() && long_rows[j] == i) {
+      sub += next - prev;
+      j++;
+    }
+    prev = next;
+    rp[i + 1] = next - sub;
+  }
+}
+
 static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
                      const std::vector<int32_t> &long_rows);
 static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
-                     const std::vector<int32_t> &long_rows, InputOnDevice &in);
+                     const std::vector<int32_t> &long_rows, const LongSplit &ls, InputOnDevice &in);
 
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int m = d->m, n = d->n;
@@ -665,11 +699,14 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     if ((int)long_rows.size() > kMaxLong) long_rows.resize(kMaxLong);  // ties at the cut
   }
   P->nlong = (int)long_rows.size();
+  // kernels 1-3: the regular row pointer, while the upload may still run
+  LongSplit ls;
+  if (d->kernel != SPX_KERNEL_SEG) split_long_rows(rp, long_rows, ls);
 
   if (upload.joinable()) upload.join();
   if (up_rc != SPX_OK) return fail(up_rc, up_msg);
   if (d->kernel == SPX_KERNEL_SEG) return build_seg(P, d, rp, long_rows);
-  return build_csr(P, d, rp, long_rows, in);
+  return build_csr(P, d, rp, long_rows, ls, in);
 }
 
 // Column bands for kernel 4's regular rows.  A unit of rows whose columns
@@ -956,21 +993,29 @@ static int build_long_slices(spx_plan *P, int n, int32_t *lcol, double *lval, co
   std::vector<int32_t> cut((size_t)(per * nlong));
   HIP_TRY(hipMemcpy(cut.data(), d_cut.p, sizeof(int32_t) * cut.size(), hipMemcpyDeviceToHost));
 
-  // slices, block-major
-  std::vector<int32_t> sb, slen, srow;
+  // slices, block-major with rows ascending in a block: counted, then
+  // placed, walking each row's cuts in order
   std::vector<int64_t> bstart(nb + 1, 0);  // slices of block b: [bstart[b], bstart[b + 1])
-  for (int b = 0; b < nb; b++) {
-    bstart[b] = (int64_t)sb.size();
+  for (int r = 0; r < nlong; r++) {
+    const int32_t *c = cut.data() + (size_t)r * per;
+    for (int b = 0; b < nb; b++) bstart[b + 1] += c[b + 1] > c[b] ? 1 : 0;
+  }
+  for (int b = 0; b < nb; b++) bstart[b + 1] += bstart[b];
+  const int64_t ns = bstart[nb];
+  std::vector<int32_t> sb((size_t)ns), slen((size_t)ns), srow((size_t)ns);
+  {
+    std::vector<int64_t> at(bstart.begin(), bstart.end() - 1);
     for (int r = 0; r < nlong; r++) {
-      const int32_t s0 = cut[(size_t)(r * per + b)], s1 = cut[(size_t)(r * per + b + 1)];
-      if (s1 <= s0) continue;
-      sb.push_back(s0);
-      slen.push_back(s1 - s0);
-      srow.push_back(r);
+      const int32_t *c = cut.data() + (size_t)r * per;
+      for (int b = 0; b < nb; b++) {
+        if (c[b + 1] <= c[b]) continue;
+        const size_t k = (size_t)at[b]++;
+        sb[k] = c[b];
+        slen[k] = c[b + 1] - c[b];
+        srow[k] = r;
+      }
     }
   }
-  bstart[nb] = (int64_t)sb.size();
-  const int64_t ns = (int64_t)sb.size();
   P->nlseg = ns;
   std::vector<int32_t> lsp(nlong + 1, 0), lseg((size_t)ns);
   for (int32_t r : srow) lsp[r + 1]++;
@@ -1055,7 +1100,7 @@ static int build_long_slices(spx_plan *P, int n, int32_t *lcol, double *lval, co
 // upload and a few passes at HBM speed; the host touches only the row
 // pointer.
 static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
-                     const std::vector<int32_t> &long_rows, InputOnDevice &in) {
+                     const std::vector<int32_t> &long_rows, const LongSplit &ls, InputOnDevice &in) {
   const int m = d->m, n = d->n;
   const int64_t nnz = d->nnz;
   const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
@@ -1063,32 +1108,13 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
   const bool f32_csr = P->precision == SPX_PREC_F32 && d->kernel != SPX_KERNEL_TILE;
   int rc;
 
-  // long rows: their input ranges in row order, and the regular row pointer
-  std::vector<LongRange> lr((size_t)P->nlong);
-  std::vector<int32_t> lrp((size_t)P->nlong + 1, 0);
-  int64_t ln = 0;
-  for (int j = 0; j < P->nlong; j++) {
-    const int r = long_rows[j];
-    lr[j] = LongRange{rp[r], rp[r + 1], ln};
-    ln += rp[r + 1] - rp[r];
-    lrp[j + 1] = (int32_t)ln;
-  }
+  // long rows split off (rp is already the regular part's row pointer)
+  const std::vector<LongRange> &lr = ls.lr;
+  const std::vector<int32_t> &lrp = ls.lrp;
+  const int64_t ln = ls.ln;
   P->long_nnz = ln;
   const int64_t nnzr = nnz - ln;
   P->nnz_reg = nnzr;
-  if (P->nlong > 0) {
-    std::vector<int32_t> rrp((size_t)m + 1, 0);
-    size_t j = 0;
-    int64_t sub = 0;
-    for (int i = 0; i < m; i++) {
-      if (j < long_rows.size() && long_rows[j] == i) {
-        sub += rp[i + 1] - rp[i];
-        j++;
-      }
-      rrp[i + 1] = (int32_t)(rp[i + 1] - sub);
-    }
-    rp.swap(rrp);
-  }
 
   // ---- the regular CSR on the device, padded by two zero entries ----
   // the input's columns and values in device memory: the caller's (device
