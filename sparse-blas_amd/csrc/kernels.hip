@@ -312,18 +312,18 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ pa
 }
 
 // ---- kernel 1: CSR-vector, L lanes per row --------------------------------------
-// Bell & Garland's CSR-vector for wave64: a group of L lanes sums a row, and
-// a group holds R rows at once.  A unit is G*R consecutive rows of one
-// partition.  When the unit has at most kBlock*kVecItems nonzeros (the usual
-// case: the lane count follows the mean row length), its nonzeros are loaded
-// with coalesced loads that need no row bounds, so they are requested at
-// once with the x window and the row bounds; the products go to LDS and
-// each row's L lanes add theirs from there.  A larger unit's lanes load their
-// rows directly: pairs of neighbouring nonzeros (one 16-byte value load, one
-// 8-byte column load) from the even index at or below the row start, 2L
-// nonzeros per step.  The group's sum is reduced with DPP lane permutations.
-// The partition's split rows go to its carry slots.  The matrix arrays are
-// padded by two entries, so a pair load past the last nonzero is readable.
+// Bell & Garland's CSR-vector for wave64: a group of L lanes sums a row.  A
+// unit is up to kVecUnitRows consecutive rows of one partition holding at
+// most kBlock*kVecItems nonzeros (the plan cuts units by nonzeros, so units
+// fill about the same loads whatever the row lengths; a small matrix gets
+// smaller units, enough to spread over the CUs).  The unit's
+// nonzeros are loaded with coalesced loads that need no row bounds, so they
+// are requested at once with the x window and the unit's row pointer, both
+// copied to LDS by the load units; the products go to LDS and each row's L
+// lanes add theirs from there, G = kBlock/L rows at a time.  A row longer
+// than that is a unit alone: the whole workgroup sums it with coalesced
+// loads.  The group's sum is reduced with DPP lane permutations.  The
+// partition's split rows go to its carry slots.
 
 typedef double vd2 __attribute__((ext_vector_type(2)));
 typedef float vf2 __attribute__((ext_vector_type(2)));
@@ -368,15 +368,32 @@ __device__ inline double group_sum(double s) {
   return s;
 }
 
-template <typename V, int L, int R>
-__global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
+// Kernel 1's result for local row lr of unit gu: y, or, for a row the
+// partition's bounds cut (begins: the row starts inside the partition, ends:
+// it ends inside it), the unit's carry slot 2gu (first row) or 2gu + 1.
+template <typename V>
+__device__ inline void vec_put(const KArgs<V> &a, int64_t gu, int r_lo, int lr, bool begins, bool ends, double t) {
+  const int r = r_lo + lr;
+  if (begins && ends) {
+    a.y[r] = y_update(a.alpha, a.beta, t, a.y, r);
+  } else {
+    const int64_t slot = 2 * gu + (lr == 0 ? 0 : 1);
+    a.crow[slot] = (int32_t)((uint32_t)r | (begins ? (uint32_t)kSlotFirst : 0u) | (ends ? kSlotFinal : 0u));
+    a.cval[slot] = t;
+  }
+}
+
+template <typename V, int L>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_vector(KArgs<V> a) {
   constexpr int G = kBlock / L;
   constexpr int PCAP = kBlock * kVecItems;
   constexpr int XS = (kSegStage + 2) > PCAP ? (kSegStage + 2) : PCAP;
   // the x window, then (after every gather) the unit's products
   __shared__ __attribute__((aligned(16))) double s_x[XS];
+  __shared__ int32_t s_rp[kVecUnitRows + 1];  // the unit's row pointer
+  __shared__ double s_red[kBlock / kWave];
   __shared__ int s_slot;
-  const int tid = threadIdx.x, g = tid / L, l = tid % L, lane = tid % kWave;
+  const int tid = threadIdx.x, g = tid / L, l = tid % L, lane = tid % kWave, wid = tid / kWave;
   // rows differ in length, so without the queue the XCDs take unit ranges
   // of about equal nonzeros (vxr) rather than equal unit counts
   int64_t iter = 0;
@@ -394,33 +411,17 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
     const int r_lo = desc_field(dcur, 0), r_hi = desc_field(dcur, 1), s0 = desc_field(dcur, 2),
               e1 = desc_field(dcur, 3), cmin = desc_field(dcur, 4), cwin = desc_field(dcur, 5),
               z0 = desc_field(dcur, 6), z1 = desc_field(dcur, 7);
-    const int nzu = z1 - z0;
-    const bool stg = cwin > 0;
-    // via: the unit's products are formed with coalesced loads and staged
-    // in LDS, where each row's lanes sum them (the loads do not wait for
-    // the row bounds); else each row's lanes load their own nonzeros
-    const bool via = nzu <= PCAP;
-    int lead = 0;
-    if (stg || via) __syncthreads();  // the previous unit is done with s_x
-    if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
-    // row bounds, clamped to the partition; fp bit 0: the row begins in the
-    // partition, bit 1: it ends in it
-    int b[R], e[R], fp[R];
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-      const int r = r_lo + k * G + g;
-      b[k] = 0;
-      e[k] = 0;
-      fp[k] = 3;
-      if (r < r_hi) {
-        const int rb = a.rowptr[r], re = a.rowptr[r + 1];
-        b[k] = rb > s0 ? rb : s0;
-        e[k] = re < e1 ? re : e1;
-        fp[k] = (rb >= s0 ? 1 : 0) | (re <= e1 ? 2 : 0);
-      }
-    }
-    double sum[R];
-    if (via) {
+    const int nr = r_hi - r_lo, nzu = z1 - z0;
+    __syncthreads();  // the previous unit is done with s_x, s_rp and s_red
+    if (nzu <= PCAP) {
+      const bool stg = cwin > 0;
+      int lead = 0;
+      if (stg) lead = stage_window<kBlock>(a.x, cmin, cwin, s_x, tid);
+      // the unit's row pointer rowptr[r_lo .. r_hi] into s_rp
+      for (int q0 = wid * kWave; q0 <= nr; q0 += kBlock)
+        if (q0 + lane <= nr)
+          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(a.rowptr + r_lo + q0 + lane),
+                                           (__attribute__((address_space(3))) void *)(s_rp + q0), 4, 0, 0);
       int ci[kVecItems];
       V vv[kVecItems];
       __builtin_amdgcn_s_setprio(2);
@@ -431,10 +432,10 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
         vv[i] = k < nzu ? __builtin_nontemporal_load(a.val + z0 + k) : (V)0;
       }
       __builtin_amdgcn_s_setprio(0);
+      stage_wait();
+      __syncthreads();  // the window and the row pointer are in LDS
       double pv[kVecItems];
       if (stg) {
-        stage_wait();
-        __syncthreads();  // the window is in LDS
 #pragma unroll
         for (int i = 0; i < kVecItems; i++)
           pv[i] = prod(vv[i], s_x[lead + (tid + i * kBlock < nzu ? ci[i] - cmin : 0)]);
@@ -449,85 +450,46 @@ __global__ __launch_bounds__(kBlock) void k_vector(KArgs<V> a) {
         if (k < nzu) s_x[k] = pv[i];
       }
       __syncthreads();  // the unit's products are in LDS
-#pragma unroll
-      for (int k = 0; k < R; k++) {
+      for (int k0 = 0; k0 < nr; k0 += G) {
+        const int lr = k0 + g;
         double t = 0.0;
-        for (int j = b[k] - z0 + l; j < e[k] - z0; j += L) t += s_x[j];
-        sum[k] = t;
+        int rb = 0, re = 0;
+        if (lr < nr) {
+          rb = s_rp[lr];
+          re = s_rp[lr + 1];
+          const int b = rb > s0 ? rb : s0, e = re < e1 ? re : e1;
+          for (int j = b - z0 + l; j < e - z0; j += L) t += s_x[j];
+        }
+        t = group_sum<L>(t);
+        if (l == 0 && lr < nr) vec_put(a, gu, r_lo, lr, rb >= s0, re <= e1, t);
       }
     } else {
-      if (stg) {
-        stage_wait();
-        __syncthreads();  // every wave's part of the window is in LDS
+      // one row longer than the unit's load capacity: every thread takes
+      // every kBlock-th nonzero, gathering from x
+      const int rb = a.rowptr[r_lo], re = a.rowptr[r_lo + 1];
+      const int b = rb > s0 ? rb : s0, e = re < e1 ? re : e1;
+      double t = 0.0;
+      int k = b + tid;
+      for (; k + 3 * kBlock < e; k += 4 * kBlock) {
+        int c[4];
+        V v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          c[i] = __builtin_nontemporal_load(a.col + k + i * kBlock);
+          v[i] = __builtin_nontemporal_load(a.val + k + i * kBlock);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) t += prod(v[i], a.x[c[i]]);
       }
-      // first step for all R rows at once: pair (b >> 1) + l covers
-      // elements 2p, 2p+1; products outside [b, e) are dropped and their
-      // gathers read the window's (or x's) first entry.  The steps are
-      // compiled twice, gathering from LDS or from x, so neither becomes a
-      // generic (flat) load.
-      auto steps = [&](auto gx) {
-        V v0[R], v1[R];
-        vi2 c[R];
+      for (; k < e; k += kBlock) t += prod(a.val[k], a.x[a.col[k]]);
+      t = group_sum<kWave>(t);
+      if (lane == 0) s_red[wid] = t;
+      __syncthreads();
+      if (tid == 0) {
+        double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-          const int p = (b[k] >> 1) + l;
-          const int q = 2 * p < e[k] ? p : (b[k] >> 1);
-          Pair<V>::load(a.val, 2 * (int64_t)q, v0[k], v1[k]);
-          c[k] = __builtin_nontemporal_load(reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)q));
-        }
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-          const int j = 2 * ((b[k] >> 1) + l);
-          const bool ok0 = j >= b[k] && j < e[k], ok1 = j + 1 >= b[k] && j + 1 < e[k];
-          const double t0 = prod(v0[k], gx(c[k].x, ok0));
-          const double t1 = prod(v1[k], gx(c[k].y, ok1));
-          sum[k] = (ok0 ? t0 : 0.0) + (ok1 ? t1 : 0.0);
-        }
-        // rows longer than the first step: four steps' loads at a time (a
-        // step past the row's end reloads the block's first pair), added in
-        // step order
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-          for (int p = (b[k] >> 1) + l + L; 2 * p < e[k]; p += 4 * L) {
-            V v0[4], v1[4];
-            vi2 c[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int q = 2 * (p + u * L) < e[k] ? p + u * L : p;
-              Pair<V>::load(a.val, 2 * (int64_t)q, v0[u], v1[u]);
-              c[u] = __builtin_nontemporal_load(reinterpret_cast<const vi2 *>(a.col + 2 * (int64_t)q));
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int q = p + u * L;
-              if (2 * q < e[k]) {
-                const bool ok1 = 2 * q + 1 < e[k];
-                const double t0 = prod(v0[u], gx(c[u].x, true)), t1 = prod(v1[u], gx(c[u].y, ok1));
-                sum[k] += t0 + (ok1 ? t1 : 0.0);
-              }
-            }
-          }
-        }
-      };
-      if (stg)
-        steps([&](int c, bool ok) -> double { return s_x[lead + (ok ? c - cmin : 0)]; });
-      else
-        steps([&](int c, bool ok) -> double { return a.x[ok ? c : 0]; });
-    }
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-      const double t = group_sum<L>(sum[k]);
-      const int r = r_lo + k * G + g;
-      if (l == 0 && r < r_hi) {
-        if (fp[k] == 3) {
-          a.y[r] = y_update(a.alpha, a.beta, t, a.y, r);
-        } else {
-          // a row cut by the partition bounds: the unit's first or last row
-          const int64_t slot = 2 * gu + (r == r_lo ? 0 : 1);
-          a.crow[slot] = (int32_t)((uint32_t)r | ((fp[k] & 1) ? (uint32_t)kSlotFirst : 0u) |
-                                   ((fp[k] & 2) ? kSlotFinal : 0u));
-          a.cval[slot] = t;
-        }
+        for (int w = 0; w < kBlock / kWave; w++) s += s_red[w];
+        vec_put(a, gu, r_lo, 0, rb >= s0, re <= e1, s);
       }
     }
   }
@@ -1139,16 +1101,16 @@ static int launch_grid(int want, int cap) {
 
 template <typename V>
 static hipError_t vec_dispatch(const KArgs<V> &a, int lanes, int grid, hipStream_t st) {
-#define SPX_VEC(L, R)                                                                                   \
-  hipLaunchKernelGGL((k_vector<V, L, R>), dim3(launch_grid(grid, RESIDENT((k_vector<V, L, R>)))), dim3(kBlock), \
+#define SPX_VEC(L)                                                                                   \
+  hipLaunchKernelGGL((k_vector<V, L>), dim3(launch_grid(grid, RESIDENT((k_vector<V, L>)))), dim3(kBlock), \
                      0, st, a)
   switch (lanes) {
-    case 2: SPX_VEC(2, kVecRows(2)); break;
-    case 4: SPX_VEC(4, kVecRows(4)); break;
-    case 8: SPX_VEC(8, kVecRows(8)); break;
-    case 16: SPX_VEC(16, kVecRows(16)); break;
-    case 32: SPX_VEC(32, kVecRows(32)); break;
-    case 64: SPX_VEC(64, kVecRows(64)); break;
+    case 2: SPX_VEC(2); break;
+    case 4: SPX_VEC(4); break;
+    case 8: SPX_VEC(8); break;
+    case 16: SPX_VEC(16); break;
+    case 32: SPX_VEC(32); break;
+    case 64: SPX_VEC(64); break;
     default: return hipErrorInvalidValue;
   }
 #undef SPX_VEC

@@ -1246,6 +1246,7 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
     tiles_per_unit = 4 * std::min<int64_t>(kTileUnitChunks, std::max<int64_t>(1, tiles / (4 * 1024)));
   }
   int64_t uoff = 0, cooff = 0, toff = 0, tpoff = 0;
+  std::vector<int32_t> vstart;  // kernel 1: first row of each unit
   for (auto &q : P->parts) {
     const int64_t rows = (int64_t)q.row_hi - q.row_lo;
     const int64_t nzq = q.e - q.s + 1;
@@ -1253,8 +1254,27 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
     q.carry_off = 2 * uoff;
     q.cslot1 = -1;
     if (P->kernel == SPX_KERNEL_VECTOR) {
-      const int64_t G = (int64_t)(kBlock / P->lanes) * kVecRows(P->lanes);
-      q.units = (rows + G - 1) / G;
+      // rows in order while the unit holds at most rcap rows (8 passes of
+      // the kBlock / lanes row groups, at most kVecUnitRows) and cap
+      // nonzeros of this partition (kBlock * kVecItems, less for a matrix
+      // too small to fill about 2,048 such units); a longer row alone
+      const int64_t rcap = std::min<int64_t>(kVecUnitRows, 8 * (kBlock / P->lanes));
+      const int64_t cap = std::min<int64_t>((int64_t)kBlock * kVecItems, std::max<int64_t>(256, (nnzr + 2047) / 2048));
+      const int64_t s0 = q.s, e1 = q.e + 1;
+      const size_t first = vstart.size();
+      for (int64_t r = q.row_lo; r < q.row_hi;) {
+        vstart.push_back((int32_t)r);
+        const int64_t r0 = r;
+        int64_t nz = 0;
+        while (r < q.row_hi && r - r0 < rcap) {
+          const int64_t len = std::min<int64_t>(rp[r + 1], e1) - std::max<int64_t>(rp[r], s0);
+          if (r > r0 && nz + len > cap) break;
+          nz += len;
+          r++;
+          if (nz > cap) break;
+        }
+      }
+      q.units = (int64_t)(vstart.size() - first);
     } else if (P->kernel == SPX_KERNEL_MERGE) {
       const int64_t L = (rows - q.eflag) + nzq;
       q.units = (L + tw - 1) / tw;
@@ -1281,13 +1301,12 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
   if (P->kernel == SPX_KERNEL_VECTOR) {
     // rows of each unit and the partition's nonzero range; the window fields
     // first carry the unit's nonzero range
-    const int64_t G = (int64_t)(kBlock / P->lanes) * kVecRows(P->lanes);
     std::vector<VUnit> vu(std::max<int64_t>(1, P->nunits));
     for (const Part &q : P->parts)
       for (int64_t u = 0; u < q.units; u++) {
         VUnit &v = vu[q.unit_off + u];
-        v.r_lo = (int32_t)(q.row_lo + u * G);
-        v.r_hi = (int32_t)std::min<int64_t>(q.row_hi, v.r_lo + G);
+        v.r_lo = vstart[(size_t)(q.unit_off + u)];
+        v.r_hi = u + 1 < q.units ? vstart[(size_t)(q.unit_off + u + 1)] : q.row_hi;
         v.s = (int32_t)q.s;
         v.e1 = (int32_t)(q.e + 1);
         v.cmin = v.z0 = std::max<int32_t>(rp[v.r_lo], v.s);

@@ -100,9 +100,10 @@ struct TUnit {
 enum { kTuSplitHead = 1, kTuSplitTail = 2 };
 constexpr int kTileUnitChunks = 4;  // chunks of 4 tiles per kernel-3 unit
 constexpr int kTileOut = 512;       // row sums per chunk gathered in LDS before they are stored
-// Kernel 1 with L lanes per row holds kVecRows(L) rows per lane group, so a
-// unit is kBlock / L * kVecRows(L) rows.
-constexpr int kVecRows(int L) { return L <= 8 ? 4 : (L == 16 ? 2 : 1); }
+// Kernel 1's units: consecutive rows of one partition, at most kVecUnitRows
+// of them and at most kBlock * kVecItems nonzeros, except a longer row, which
+// is a unit alone (the plan may cap both lower).
+constexpr int kVecUnitRows = 1024;
 
 // Carry slots hold pieces of rows split between units, in nonzero order.  A
 // slot's row word is -1 when unused, else the row with two flag bits: the
