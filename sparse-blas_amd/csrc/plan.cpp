@@ -592,10 +592,41 @@ static void split_long_rows(std::vector<int32_t> &rp, const std::vector<int32_t>
   }
 }
 
+// Kernel 1's lanes per row: the caller's, or from the mean row length.
+static int vec_lanes(const spx_plan_desc *d, int64_t nnzr) {
+  return d->lanes > 0 ? d->lanes : default_lanes(d->m > 0 ? (double)nnzr / d->m : 0);
+}
+
+// Kernel 1's units over rows [lo, hi) of a partition holding nonzeros
+// [s0, e1) of a regular part with nnzr nonzeros: the first row of each is
+// appended to out.  A unit takes rows in order while it holds at most rcap
+// rows (8 passes of the kBlock / lanes row groups, at most kVecUnitRows) and
+// cap nonzeros of the partition (kBlock * kVecItems, less for a matrix too
+// small to fill about 2,048 such units); a longer row is a unit alone.  Rows
+// [r0, r) hold min(rp[r], e1) - max(rp[r0], s0) nonzeros, which grows with
+// r, so each unit's end is found by binary search.
+static void vec_units(const std::vector<int32_t> &rp, int64_t lo, int64_t hi, int64_t s0, int64_t e1, int lanes,
+                      int64_t nnzr, std::vector<int32_t> &out) {
+  const int64_t rcap = std::min<int64_t>(kVecUnitRows, 8 * std::max(1, kBlock / lanes));
+  const int64_t cap = std::min<int64_t>((int64_t)kBlock * kVecItems, std::max<int64_t>(256, (nnzr + 2047) / 2048));
+  for (int64_t r0 = lo; r0 < hi;) {
+    out.push_back((int32_t)r0);
+    const int64_t rmax = std::min<int64_t>(hi, r0 + rcap);
+    const int64_t limit = cap + std::max<int64_t>(rp[r0], s0);
+    int64_t r = rmax;
+    if (e1 > limit)
+      r = (int64_t)(std::upper_bound(rp.begin() + r0 + 1, rp.begin() + rmax + 1,
+                                     (int32_t)std::min<int64_t>(limit, INT32_MAX)) -
+                    rp.begin()) - 1;
+    r0 = r > r0 ? r : r0 + 1;
+  }
+}
+
 static int build_seg(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
                      const std::vector<int32_t> &long_rows);
 static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
-                     const std::vector<int32_t> &long_rows, const LongSplit &ls, InputOnDevice &in);
+                     const std::vector<int32_t> &long_rows, const LongSplit &ls,
+                     const std::vector<int32_t> &vwhole, InputOnDevice &in);
 
 static int plan_build(spx_plan *P, const spx_plan_desc *d) {
   const int m = d->m, n = d->n;
@@ -699,14 +730,21 @@ static int plan_build(spx_plan *P, const spx_plan_desc *d) {
     if ((int)long_rows.size() > kMaxLong) long_rows.resize(kMaxLong);  // ties at the cut
   }
   P->nlong = (int)long_rows.size();
-  // kernels 1-3: the regular row pointer, while the upload may still run
+  // kernels 1-3: the regular row pointer, while the upload may still run;
+  // kernel 1 also cuts the units of a plan with one partition (all rows),
+  // while the row pointer is in the host's caches
   LongSplit ls;
+  std::vector<int32_t> vwhole;
   if (d->kernel != SPX_KERNEL_SEG) split_long_rows(rp, long_rows, ls);
+  if (d->kernel == SPX_KERNEL_VECTOR) {
+    const int lanes = vec_lanes(d, nnz - ls.ln);
+    if (lanes >= 1 && lanes <= 64 && !(lanes & (lanes - 1))) vec_units(rp, 0, m, 0, nnz - ls.ln, lanes, nnz - ls.ln, vwhole);
+  }
 
   if (upload.joinable()) upload.join();
   if (up_rc != SPX_OK) return fail(up_rc, up_msg);
   if (d->kernel == SPX_KERNEL_SEG) return build_seg(P, d, rp, long_rows);
-  return build_csr(P, d, rp, long_rows, ls, in);
+  return build_csr(P, d, rp, long_rows, ls, vwhole, in);
 }
 
 // Column bands for kernel 4's regular rows.  A unit of rows whose columns
@@ -1100,7 +1138,8 @@ static int build_long_slices(spx_plan *P, int n, int32_t *lcol, double *lval, co
 // upload and a few passes at HBM speed; the host touches only the row
 // pointer.
 static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &rp,
-                     const std::vector<int32_t> &long_rows, const LongSplit &ls, InputOnDevice &in) {
+                     const std::vector<int32_t> &long_rows, const LongSplit &ls,
+                     const std::vector<int32_t> &vwhole, InputOnDevice &in) {
   const int m = d->m, n = d->n;
   const int64_t nnz = d->nnz;
   const bool on_dev = (d->flags & SPX_FLAG_DEVICE_PTRS) != 0;
@@ -1221,9 +1260,8 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
   }
 
   // ---- per-kernel work decomposition ----
-  const double mean = m > 0 ? (double)nnzr / m : 0;
   if (P->kernel == SPX_KERNEL_VECTOR) {
-    P->lanes = d->lanes > 0 ? d->lanes : default_lanes(mean);
+    P->lanes = vec_lanes(d, nnzr);
     if (P->lanes > 64 || (P->lanes & (P->lanes - 1))) return fail(SPX_EARG, "lanes must be a power of two <= 64");
   } else if (P->kernel == SPX_KERNEL_MERGE) {
     P->items = d->items > 0 ? d->items : 8;
@@ -1254,26 +1292,11 @@ static int build_csr(spx_plan *P, const spx_plan_desc *d, std::vector<int32_t> &
     q.carry_off = 2 * uoff;
     q.cslot1 = -1;
     if (P->kernel == SPX_KERNEL_VECTOR) {
-      // rows in order while the unit holds at most rcap rows (8 passes of
-      // the kBlock / lanes row groups, at most kVecUnitRows) and cap
-      // nonzeros of this partition (kBlock * kVecItems, less for a matrix
-      // too small to fill about 2,048 such units); a longer row alone
-      const int64_t rcap = std::min<int64_t>(kVecUnitRows, 8 * (kBlock / P->lanes));
-      const int64_t cap = std::min<int64_t>((int64_t)kBlock * kVecItems, std::max<int64_t>(256, (nnzr + 2047) / 2048));
-      const int64_t s0 = q.s, e1 = q.e + 1;
       const size_t first = vstart.size();
-      for (int64_t r = q.row_lo; r < q.row_hi;) {
-        vstart.push_back((int32_t)r);
-        const int64_t r0 = r;
-        int64_t nz = 0;
-        while (r < q.row_hi && r - r0 < rcap) {
-          const int64_t len = std::min<int64_t>(rp[r + 1], e1) - std::max<int64_t>(rp[r], s0);
-          if (r > r0 && nz + len > cap) break;
-          nz += len;
-          r++;
-          if (nz > cap) break;
-        }
-      }
+      if (P->parts.size() == 1 && q.row_lo == 0 && q.row_hi == m && q.s == 0 && q.e + 1 == nnzr && !vwhole.empty())
+        vstart = vwhole;  // cut before the upload ended
+      else
+        vec_units(rp, q.row_lo, q.row_hi, q.s, q.e + 1, P->lanes, nnzr, vstart);
       q.units = (int64_t)(vstart.size() - first);
     } else if (P->kernel == SPX_KERNEL_MERGE) {
       const int64_t L = (rows - q.eflag) + nzq;
