@@ -316,37 +316,14 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ pa
 // unit is up to kVecUnitRows consecutive rows of one partition holding at
 // most kBlock*kVecItems nonzeros (the plan cuts units by nonzeros, so units
 // fill about the same loads whatever the row lengths; a small matrix gets
-// smaller units, enough to spread over the CUs).  The unit's
-// nonzeros are loaded with coalesced loads that need no row bounds, so they
-// are requested at once with the x window and the unit's row pointer, both
-// copied to LDS by the load units; the products go to LDS and each row's L
-// lanes add theirs from there, G = kBlock/L rows at a time.  A row longer
-// than that is a unit alone: the whole workgroup sums it with coalesced
-// loads.  The group's sum is reduced with DPP lane permutations.  The
-// partition's split rows go to its carry slots.
-
-typedef double vd2 __attribute__((ext_vector_type(2)));
-typedef float vf2 __attribute__((ext_vector_type(2)));
-typedef int vi2 __attribute__((ext_vector_type(2)));
-
-template <typename V>
-struct Pair;
-template <>
-struct Pair<double> {
-  __device__ static inline void load(const double *p, int64_t k, double &a, double &b) {
-    const vd2 u = __builtin_nontemporal_load(reinterpret_cast<const vd2 *>(p + k));
-    a = u.x;
-    b = u.y;
-  }
-};
-template <>
-struct Pair<float> {
-  __device__ static inline void load(const float *p, int64_t k, float &a, float &b) {
-    const vf2 u = __builtin_nontemporal_load(reinterpret_cast<const vf2 *>(p + k));
-    a = u.x;
-    b = u.y;
-  }
-};
+// smaller units, enough to spread over the CUs).  The unit's nonzeros are
+// loaded with coalesced loads that need no row bounds, so they are requested
+// at once with the x window and the unit's row pointer, both copied to LDS
+// by the load units; the products go to LDS and each row's L lanes add
+// theirs from there, G = kBlock/L rows at a time.  A row of more than
+// kBlock*kVecItems nonzeros is a unit alone: the whole workgroup sums it
+// with coalesced loads.  The group's sum is reduced with DPP lane
+// permutations.  The partition's split rows go to its carry slots.
 
 // One DPP permutation of a double (all lanes have a source for these controls).
 template <int CTRL>
