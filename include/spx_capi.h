@@ -155,7 +155,9 @@ int spx_plan_time(spx_plan_t p, double alpha, const double *x, double beta,
                   double *out_ms);
 /* The same timing with every execute's figures kept: exec_ms[i] and
  * kernel_ms[i] (arrays of `reps`) as out_ms[0] and [1] above, *wall_ms (may
- * be NULL) the host wall clock per execute. */
+ * be NULL) the host wall clock per execute.  With exec_ms and kernel_ms both
+ * NULL no events are recorded (each event pair costs the stream a few
+ * microseconds) and only *wall_ms is returned: the executes' own rate. */
 int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta,
                        double *y, int warmup, int reps, int64_t flush_bytes,
                        double *exec_ms, double *kernel_ms, double *wall_ms);

@@ -1762,12 +1762,15 @@ int64_t spx_plan_export(spx_plan_t p, int what, void *dst, int64_t cap) {
 
 int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta, double *y, int warmup, int reps,
                        int64_t flush_bytes, double *exec_ms, double *kernel_ms, double *wall_ms) {
-  if (!p || reps <= 0 || !exec_ms || !kernel_ms) return fail(SPX_EARG, "bad timing arguments");
-  // Every execute gets its own event pairs, so the timed executes run
-  // back-to-back on one stream with a single host wait at the end.
+  const bool events = exec_ms != nullptr;
+  if (!p || reps <= 0 || events != (kernel_ms != nullptr) || (!events && !wall_ms))
+    return fail(SPX_EARG, "bad timing arguments");
+  // Every execute gets its own event pairs (unless only the wall clock is
+  // asked for), so the timed executes run back-to-back on one stream with a
+  // single host wait at the end.
   hipStream_t st = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  std::vector<hipEvent_t> ev(4 * (size_t)reps, nullptr);
+  std::vector<hipEvent_t> ev(events ? 4 * (size_t)reps : 0, nullptr);
   int rc = SPX_OK;
   for (auto &e : ev)
     if (rc == SPX_OK && hipEventCreate(&e) != hipSuccess) rc = fail(SPX_EDEV, "hipEventCreate failed");
@@ -1786,6 +1789,10 @@ int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta,
   for (int i = 0; i < reps && rc == SPX_OK; i++) {
     if (flush && flush_bytes > 0) (void)launch_flush_read(flush, fb, sink, st);
     if (flush && flush_bytes < 0) (void)hipMemsetAsync(flush, i & 0xff, (size_t)fb, st);
+    if (!events) {
+      rc = execute_impl(p, alpha, x, beta, y, st, nullptr, nullptr, false);
+      continue;
+    }
     (void)hipEventRecord(ev[4 * i], st);
     rc = execute_impl(p, alpha, x, beta, y, st, ev[4 * i + 1], ev[4 * i + 2], false);
     (void)hipEventRecord(ev[4 * i + 3], st);
@@ -1793,7 +1800,7 @@ int spx_plan_time_each(spx_plan_t p, double alpha, const double *x, double beta,
   if (hipStreamSynchronize(st) != hipSuccess && rc == SPX_OK) rc = fail(SPX_EDEV, "stream synchronize failed");
   const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
   if (rc == SPX_OK) {
-    for (int i = 0; i < reps; i++) {
+    for (int i = 0; i < reps && events; i++) {
       float t = 0, tk = 0;
       (void)hipEventElapsedTime(&t, ev[4 * i], ev[4 * i + 3]);
       (void)hipEventElapsedTime(&tk, ev[4 * i + 1], ev[4 * i + 2]);

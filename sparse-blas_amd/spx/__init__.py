@@ -269,6 +269,14 @@ class Plan:
                                        e.ctypes.data, k.ctypes.data, C.byref(w)))
         return e, k, w.value
 
+    def time_wall(self, x_dev, y_dev, alpha=1.0, beta=0.0, warmup=5, reps=20):
+        """Host wall-clock ms per execute of `reps` back-to-back executes with no
+        events between them (synchronized before and after)."""
+        w = C.c_double()
+        check(lib().spx_plan_time_each(self.h, alpha, x_dev.ptr, beta, y_dev.ptr, warmup, reps, 0,
+                                       None, None, C.byref(w)))
+        return w.value
+
     def close(self):
         if getattr(self, "h", None):
             lib().spx_plan_destroy(self.h)

@@ -517,3 +517,23 @@ def test_kernel4_x_at_either_16_byte_phase(offset, precision):
     else:
         ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, 0.5, -2.0, y0)
         assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_plan_timing_with_and_without_events():
+    """spx_plan_time_each records HIP events around every execute and its SpMV
+    kernel, or, with both per-execute arrays NULL, none (wall clock only);
+    one array without the other is an argument error.  The executes still
+    produce y either way."""
+    A = W.circuit(m=100_000, total=1_000_000, n_dense=4, window=500, seed=9)
+    x = W.vector(A["n"], 41)
+    print(f"timing case nnz={A['nnz']}", flush=True)
+    P = spx.Plan(A["rowptr"], A["col"], A["val"], n=A["n"], kernel=4, items=2)
+    xd, yd = spx.DeviceBuffer.from_array(x), spx.DeviceBuffer(A["m"] * 8)
+    w = P.time_wall(xd, yd, 1.0, 0.0, warmup=1, reps=5)
+    assert np.isfinite(w) and w > 0
+    check_fp64(A, x, yd.download(np.float64, A["m"]), 1.0, 0.0, None)
+    e, k, w2 = P.time_each(xd, yd, 1.0, 0.0, warmup=1, reps=5)
+    assert np.all(e > 0) and np.all(k > 0) and np.all(k <= e) and w2 > 0
+    only = np.zeros(5)
+    rc = spx.lib().spx_plan_time_each(P.h, 1.0, xd.ptr, 0.0, yd.ptr, 0, 5, 0, only.ctypes.data, None, None)
+    assert rc == spx.EARG
