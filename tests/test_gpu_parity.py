@@ -421,7 +421,7 @@ def test_c2_circuit_full_size(c2, kc):
 
 
 @pytest.mark.parametrize("name", ["rail", "asic"])
-@pytest.mark.parametrize("kernel", [2, 3, 4])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 def test_precision_form(name, kernel):
     A = W.rail() if name == "rail" else W.asic()
     x = W.vector(A["n"], 31)
@@ -479,7 +479,7 @@ def test_long_row_path_precision_form():
     A = W.zipf(6000, 1.1, 3000, seed=22)
     x = W.vector(A["n"], 7)
     ref = oracle.spmv(A["rowptr"], A["col"], A["val"], x, 1.0, 0.0)
-    for kernel in (2, 4):
+    for kernel in (1, 2, 4):
         y, P = run_plan(A, x, kernel=kernel, long_row=100, precision=spx.PREC_F32)
         assert P.info()["nlong"] > 0
         assert np.linalg.norm(y - ref) / np.linalg.norm(ref) <= 1e-7
@@ -537,3 +537,42 @@ def test_plan_timing_with_and_without_events():
     only = np.zeros(5)
     rc = spx.lib().spx_plan_time_each(P.h, 1.0, xd.ptr, 0.0, yd.ptr, 0, 5, 0, only.ctypes.data, None, None)
     assert rc == spx.EARG
+
+
+def _kernel1_shapes():
+    """Rows around kernel 1's unit capacity (2,048 nonzeros), a run of empty
+    rows longer than its row cap (1,024), and short rows between them."""
+    rng = np.random.default_rng(17)
+    lens = np.concatenate([[3000], np.zeros(1500, np.int64), rng.integers(1, 5, 3000), [2500, 2049, 2048, 2047],
+                           np.zeros(100, np.int64), np.full(500, 7), [4000], np.zeros(1030, np.int64), [1]])
+    n = 9000
+    rp = np.zeros(len(lens) + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, int(k), replace=False)) for k in lens if k > 0]).astype(np.int32)
+    val = rng.uniform(-1, 1, int(rp[-1]))
+    return W.finish(rp, col, val, len(lens), n)
+
+
+K1_SHAPES = _kernel1_shapes()
+
+
+@pytest.mark.parametrize("precision", [spx.PREC_F64, spx.PREC_F32])
+@pytest.mark.parametrize("lanes", [0, 2, 64])
+@pytest.mark.parametrize("mc", [dict(parts=1), dict(parts=3), dict(mode=spx.MODE_QUEUE, nb=2600, queues=2)],
+                         ids=["p1", "p3", "queue"])
+def test_kernel1_unit_shapes(mc, lanes, precision):
+    """Kernel 1 cuts a partition's rows into units of at most 2,048 nonzeros
+    and at most 8 passes of its row groups (1,024 rows): a longer row is a
+    unit alone, summed by the whole workgroup; a run of empty rows fills
+    units up to the row cap; partition bounds cut rows of either kind."""
+    A = K1_SHAPES
+    x = W.vector(A["n"], 71)
+    y0 = W.vector(A["m"], 72)
+    print(f"kernel-1 shapes lanes={lanes} precision={precision} {mc}", flush=True)
+    y, P = run_plan(A, x, 0.75, -1.5, y0, kernel=1, lanes=lanes, precision=precision, **mc)
+    assert P.info()["nlong"] == 0
+    if precision == spx.PREC_F64:
+        check_fp64(A, x, y, 0.75, -1.5, y0)
+    else:
+        ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, 0.75, -1.5, y0)
+        assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
