@@ -576,3 +576,52 @@ def test_kernel1_unit_shapes(mc, lanes, precision):
     else:
         ref = oracle.spmv_f32val(A["rowptr"], A["col"], A["val"].astype(np.float32), x, 0.75, -1.5, y0)
         assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def _random_matrix(seed):
+    """A seeded random CSR matrix: one of four row-length laws (short uniform,
+    heavy-tailed, mostly empty, a few rows far beyond the rest), columns
+    distinct and sorted per row, values U[-1, 1)."""
+    rng = np.random.default_rng(1000 + seed)
+    m = int(rng.integers(1, 20000))
+    n = int(rng.integers(1, 30000))
+    law = seed % 4
+    if law == 0:
+        lens = rng.integers(0, 9, m)
+    elif law == 1:
+        lens = np.minimum(rng.zipf(1.4, m), 6000)
+    elif law == 2:
+        lens = (rng.random(m) < 0.05) * rng.integers(1, 40, m)
+    else:
+        lens = rng.integers(0, 4, m)
+        big = rng.choice(m, min(m, int(rng.integers(1, 6))), replace=False)
+        lens[big] = rng.integers(2000, 9000, len(big))
+    lens = np.minimum(lens, n).astype(np.int64)
+    rp = np.zeros(m + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, int(k), replace=False)) for k in lens if k > 0]).astype(np.int32) \
+        if rp[-1] else np.zeros(0, np.int32)
+    return W.finish(rp, col, rng.uniform(-1, 1, int(rp[-1])), m, n)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_matrices_all_kernels(seed):
+    """Every kernel, at its default schedule and at a seeded random one
+    (partition count or task size), on seeded random matrices against the
+    oracle's per-row bound."""
+    A = _random_matrix(seed)
+    rng = np.random.default_rng(seed)
+    x = W.vector(max(A["n"], 1), 200 + seed)[: A["n"]]
+    y0 = W.vector(A["m"], 300 + seed)
+    print(f"random matrix seed={seed} m={A['m']} n={A['n']} nnz={A['nnz']}", flush=True)
+    for kernel in (1, 2, 3, 4):
+        y, _ = run_plan(A, x, 1.5, 0.0, None, kernel=kernel)
+        check_fp64(A, x, y, 1.5, 0.0, None)
+        if kernel == 4:
+            continue
+        if seed % 2:
+            kw = dict(mode=spx.MODE_QUEUE, nb=int(rng.integers(500, 20000)), queues=int(rng.integers(1, 4)))
+        else:
+            kw = dict(mode=spx.MODE_NNZ, parts=int(rng.integers(2, 9)))
+        y, _ = run_plan(A, x, -0.5, 0.25, y0, kernel=kernel, **kw)
+        check_fp64(A, x, y, -0.5, 0.25, y0)
