@@ -3,7 +3,7 @@ for a CSR matrix from 32 device-computed features with a small MLP
 (32 -> 64 -> 64 -> 96, FP32) whose training and inference run in the
 library's own HIP kernels.  It replaces the reference harness's exhaustive
 (d, c) search (SB/test/dspmv_test.cu:314-332); exhaustive search stays
-available (`exhaustive`) as the fallback and as the judge of regret.
+available (`exhaustive`) as the fallback and as the yardstick of regret.
 
 Schedule s in [0, 96): kernel 1 + s // 24, width index (s // 6) % 4, parts
 PARTS[s % 6] (kernel 4 ignores parts, so its six copies share one timing).
