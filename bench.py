@@ -391,8 +391,8 @@ def main():
                     help="batch: schedules of the selector's shortlist the autotuner times")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build drives one MI355X; --gpus must be 1 (the multi-GPU "
-                                   "design is described, not built)"}))
+        print(json.dumps({"error": "the library drives one MI355X; --gpus must be 1 (sharding over "
+                                   "several GPUs is designed, not implemented)"}))
         sys.exit(2)
 
     if a.workload == "batch":
